@@ -1,0 +1,128 @@
+/*
+ * sfmm — MI355X-native Laplace FMM evaluator, C-ABI.
+ *
+ * Drop-in for the evaluation path of icyspirit/SimpleFMM: the calls below are
+ * what that project's header-only C++ API (fmm.hpp, driven as in main.cpp)
+ * does, split into a host tree (Box + OctreePartitioner) and a device plan
+ * (FMM3D).  Plain pointers and sizes only; every function returns 0 on success
+ * or a non-zero status whose text sfmm_last_error() returns.
+ *
+ * Particle arrays are caller-owned, in the caller's ("original") order, AoS:
+ * positions double[3n]; charges double[qdim*n]; outputs double[nout*n].
+ * Outputs are ACCUMULATED (U += ...), never zeroed, as the reference's rinv
+ * (fmm.hpp:767,788,792,801,805).
+ */
+#ifndef SFMM_H
+#define SFMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sfmm_tree_s* sfmm_tree;
+typedef struct sfmm_plan_s* sfmm_plan;
+
+enum {
+    SFMM_OK = 0,
+    SFMM_ERR_ARG = 1,      /* invalid argument */
+    SFMM_ERR_HIP = 2,      /* HIP runtime error (no device, launch failure, ...) */
+    SFMM_ERR_ALLOC = 3,    /* device allocation failed */
+    SFMM_ERR_STATE = 4     /* call not valid in the object's state */
+};
+
+/* Evaluation kinds.  qdim/nout are per particle. */
+enum {
+    SFMM_POTENTIAL = 0, /* rinv<false>: ncharge charge components -> ncharge potentials, sum Q_j/|r| */
+    SFMM_CURL = 1,      /* rinv<true>: 3-vector charge -> sum Q_j x R/|R|^3, R = x_i - x_j          */
+    SFMM_POTGRAD = 2    /* scalar charge -> (phi, dphi/dx, dphi/dy, dphi/dz), nout = 4               */
+};
+
+/* Which part of the sum (rinv = both, rinv_nonear = far, N2N = near). */
+enum { SFMM_FAR = 1, SFMM_NEAR = 2, SFMM_BOTH = 3 };
+
+enum { SFMM_FP64 = 0, SFMM_MIXED = 1 };
+
+typedef struct {
+    int64_t n;                 /* number of particles                                 */
+    const double* positions;   /* double[3n], AoS                                     */
+    int max_level;             /* refine(max_level, ...) — at most 20                 */
+    int max_per_leaf;          /* refine(..., max_particles_per_node)                 */
+    int vertical_axis;         /* Box(..., vertical_axis, phideg)                     */
+    double phideg;
+    int box_given;             /* 0: Box(positions, ...); 1: Box(corner, width, ...)  */
+    double corner[3];
+    double width;
+    int threads;               /* host threads for the build (0 = hardware)           */
+} sfmm_tree_desc;
+
+/* Box(...) + OctreePartitioner(positions, box) + refine(max_level, cap):
+ * box.hpp:73-107, partition.hpp:397-446.  The tree copies the positions. */
+int sfmm_tree_create(const sfmm_tree_desc* desc, sfmm_tree* out);
+int sfmm_tree_destroy(sfmm_tree tree);
+
+typedef struct {
+    int depth;             /* partitioner.level()            */
+    int64_t n_node;        /* over all levels                */
+    int64_t n_leaf;
+    int64_t m2l_pairs;     /* sum of interaction list sizes  */
+    int64_t near_pairs;    /* near-field (i, j) pairs, i != j */
+    double width;          /* box.width()                    */
+    double corner[3];      /* box.corner()                   */
+} sfmm_tree_info;
+int sfmm_tree_get_info(sfmm_tree tree, sfmm_tree_info* info);
+
+/* Per-level counts: n_node, n_leaf, nnz(indices), nnz(ilist), nnz(clist), nnz(nlist). */
+int sfmm_tree_level_counts(sfmm_tree tree, int level, int64_t counts[6]);
+/* Per-level lists in the reference's layout (partition.hpp:155-166); any pointer may be NULL.
+ * ilist values: (src, di, dj, dk); clist: (child, code); plist: (parent, code); nlist: (level, leaf). */
+int sfmm_tree_level_lists(sfmm_tree tree, int level, int64_t* c_node, int64_t* leaf2node, int64_t* idx_row,
+                          int64_t* idx_val, int64_t* ilist_row, int64_t* ilist_val, int64_t* clist_row,
+                          int64_t* clist_val, int64_t* plist, int64_t* nlist_row, int64_t* nlist_val);
+
+typedef struct {
+    int p;                    /* expansion order, 1..16 (FMM3D template p)            */
+    int precision;            /* SFMM_FP64 or SFMM_MIXED                              */
+    int max_ncharge;          /* largest ncharge later passed to SFMM_POTENTIAL (1..4) */
+    /* optional self exclusions (FMM3D(partitioner, comm, self_generator), fmm.hpp:204):
+     * for particle i, pairs (i, j), j in self_val[self_row[i] .. self_row[i+1]), are
+     * excluded; NULL means j == i only. */
+    const int64_t* self_row;
+    const int64_t* self_val;
+} sfmm_plan_desc;
+
+/* FMM3D(partitioner, comm[, self_generator]): fmm.hpp:204-427.  Uploads the tree
+ * and the operator tables to the current HIP device.  The tree must outlive the plan. */
+int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out);
+int sfmm_plan_destroy(sfmm_plan plan);
+
+/* rinv / rinv<true> / rinv_nonear / N2N with host arrays (fmm.hpp:811-881):
+ * U[nout*i + c] += value, original particle order.  Synchronous. */
+int sfmm_evaluate(sfmm_plan plan, int kind, int ncharge, const double* Q, double* U, int which);
+
+/* Same with device arrays, asynchronous on the plan's stream (sfmm_plan_stream).
+ * dQ: double[qdim*n] AoS original order; dU: double[nout*n], accumulated. */
+int sfmm_evaluate_device(sfmm_plan plan, int kind, int ncharge, const double* dQ, double* dU, int which);
+
+/* The plan's HIP stream (hipStream_t) for sfmm_evaluate_device. */
+int sfmm_plan_stream(sfmm_plan plan, void** stream);
+int sfmm_synchronize(sfmm_plan plan);
+
+/* Device milliseconds of each stage of the last evaluation:
+ * [0] gather, [1] P2M, [2] M2M, [3] M2L, [4] L2L, [5] L2P, [6] P2P, [7] scatter, [8] total. */
+int sfmm_last_stage_ms(sfmm_plan plan, float ms[9]);
+
+/* FP64 direct sum over all sources j != i for the given original-order targets
+ * (main.cpp:70-96): out[nout*t + c] = value (overwritten).  Host arrays, synchronous. */
+int sfmm_direct(sfmm_plan plan, int kind, int ncharge, const double* Q, int64_t ntarget, const int64_t* targets,
+                double* out);
+
+/* Text of the last error on this thread. */
+const char* sfmm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -128,7 +128,7 @@ struct Box {
         }
         const double dmax = std::max(dl[0], std::max(dl[1], dl[2]));
         width = (1 + 2 * 1e-3) * dl[2];
-        while (width < dmax) {
+        while (width > 0 && width < dmax) {
             c[2] += width / 2;
             width *= 2;
         }
@@ -819,6 +819,10 @@ void* oracle_tree_build(int64_t n, const double* pos, int max_level, int cap, in
 {
     if (n <= 0 || !pos || max_level < 0 || max_level > 20) return nullptr;
     auto* t = new Tree(n, pos, vertical_axis, phideg);
+    if (!(t->box.width > 0)) {  // zero vertical extent: the reference cannot build this box
+        delete t;
+        return nullptr;
+    }
     Node root{0, 0, {}, {}};
     root.idx.resize(n);
     for (int64_t i = 0; i < n; ++i) root.idx[i] = static_cast<int>(i);

@@ -1,0 +1,276 @@
+"""ctypes binding of libsfmm.so and a Python mirror of the reference API.
+
+The reference (icyspirit/SimpleFMM) is a header-only C++ library driven as in
+its main.cpp:38-66:
+
+    Box<double,3,Vector> box(positions, vertical_axis);
+    OctreePartitioner<double,Vector> partitioner(positions, box);
+    partitioner.refine(max_level, max_particles_per_node);
+    FMM3D<double,3,p,Vector,true> fmm(partitioner, MPI_COMM_WORLD);
+    fmm.rinv(Q, U);  fmm.rinv<true>(Q, U);
+
+`Box`, `OctreePartitioner` and `FMM3D` below keep those names, argument
+meanings and the accumulate-into-U semantics; the work runs in libsfmm.so
+(host tree in C++17, evaluation in HIP kernels on the current GPU).  There is
+no CPU fallback: FMM3D raises if the library or the GPU is unavailable.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsfmm.so")
+
+POTENTIAL, CURL, POTGRAD = 0, 1, 2
+FAR, NEAR, BOTH = 1, 2, 3
+FP64, MIXED = 0, 1
+STAGES = ("gather", "P2M", "M2M", "M2L", "L2L", "L2P", "P2P", "scatter", "total")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int64)
+
+
+class SfmmError(RuntimeError):
+    pass
+
+
+class TreeDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("positions", _dp), ("max_level", ctypes.c_int),
+                ("max_per_leaf", ctypes.c_int), ("vertical_axis", ctypes.c_int), ("phideg", ctypes.c_double),
+                ("box_given", ctypes.c_int), ("corner", ctypes.c_double * 3), ("width", ctypes.c_double),
+                ("threads", ctypes.c_int)]
+
+
+class TreeInfo(ctypes.Structure):
+    _fields_ = [("depth", ctypes.c_int), ("n_node", ctypes.c_int64), ("n_leaf", ctypes.c_int64),
+                ("m2l_pairs", ctypes.c_int64), ("near_pairs", ctypes.c_int64), ("width", ctypes.c_double),
+                ("corner", ctypes.c_double * 3)]
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_int), ("precision", ctypes.c_int), ("max_ncharge", ctypes.c_int),
+                ("self_row", _ip), ("self_val", _ip)]
+
+
+# symbol -> (restype, argtypes); every function declared in include/sfmm.h
+SIGNATURES = {
+    "sfmm_tree_create": (ctypes.c_int, [ctypes.POINTER(TreeDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "sfmm_tree_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfmm_tree_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TreeInfo)]),
+    "sfmm_tree_level_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip]),
+    "sfmm_tree_level_lists": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_ip] * 11),
+    "sfmm_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PlanDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "sfmm_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfmm_evaluate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int]),
+    "sfmm_evaluate_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_int]),
+    "sfmm_plan_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "sfmm_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfmm_last_stage_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "sfmm_direct": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int64, _ip, _dp]),
+    "sfmm_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def build(force: bool = False) -> None:
+    """Compiles libsfmm.so in-tree (hipcc, gfx950)."""
+    args = ["make", "-s", "-C", _HERE, "-j8"]
+    if force:
+        subprocess.run(["make", "-s", "-C", _HERE, "clean"], check=True)
+    subprocess.run(args, check=True)
+
+
+def lib():
+    """Loads libsfmm.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SfmmError(f"{LIB_PATH} is missing: build it with `make -C {_HERE}` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise SfmmError(f"sfmm error {rc}: {lib().sfmm_last_error().decode()}")
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+# ----------------------------------------------------------------- reference API
+class Box:
+    """Box(positions, vertical_axis=2, phideg=0) or Box(corner=..., width=...) (box.hpp:73-107)."""
+
+    def __init__(self, positions=None, vertical_axis: int = 2, phideg: float = 0.0, *, corner=None, width=None):
+        self.vertical_axis = int(vertical_axis)
+        self.phideg = float(phideg)
+        self.given = corner is not None
+        if self.given:
+            self.corner = np.asarray(corner, dtype=np.float64).reshape(3)
+            self.width_ = float(width)
+        elif positions is None:
+            raise ValueError("Box needs positions or (corner, width)")
+
+
+class OctreePartitioner:
+    """OctreePartitioner(positions, box).refine(max_level, max_particles_per_node) (partition.hpp:380-446)."""
+
+    def __init__(self, positions, box: Box):
+        self._pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        self._box = box
+        self.h = None
+
+    def refine(self, max_level: int = 20, max_particles_per_node: int = 128, threads: int = 0):
+        desc = TreeDesc()
+        desc.n = self._pos.shape[0]
+        desc.positions = _d(self._pos)
+        desc.max_level = int(max_level)
+        desc.max_per_leaf = int(max_particles_per_node)
+        desc.vertical_axis = self._box.vertical_axis
+        desc.phideg = self._box.phideg
+        desc.box_given = int(self._box.given)
+        if self._box.given:
+            for d in range(3):
+                desc.corner[d] = self._box.corner[d]
+            desc.width = self._box.width_
+        desc.threads = int(threads)
+        h = ctypes.c_void_p()
+        _check(lib().sfmm_tree_create(ctypes.byref(desc), ctypes.byref(h)))
+        self.h = h
+        return self
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.sfmm_tree_destroy(self.h)
+            self.h = None
+
+    @property
+    def n(self) -> int:
+        return self._pos.shape[0]
+
+    def info(self) -> TreeInfo:
+        info = TreeInfo()
+        _check(lib().sfmm_tree_get_info(self.h, ctypes.byref(info)))
+        return info
+
+    def level(self) -> int:
+        return self.info().depth
+
+    def level_lists(self, l: int) -> dict:
+        c = np.zeros(6, dtype=np.int64)
+        _check(lib().sfmm_tree_level_counts(self.h, l, _i(c)))
+        n_node, n_leaf, nidx, nil, ncl, nnl = (int(x) for x in c)
+        depth = self.level()
+        out = {
+            "n_node": n_node, "n_leaf": n_leaf,
+            "c_node": np.zeros(n_node, np.int64), "leaf2node": np.zeros(n_leaf, np.int64),
+            "idx_row": np.zeros(n_leaf + 1, np.int64), "idx_val": np.zeros(nidx, np.int64),
+            "ilist_row": np.zeros((n_node + 1) if l >= 2 else 1, np.int64), "ilist_val": np.zeros(4 * nil, np.int64),
+            "clist_row": np.zeros((n_node + 1) if l < depth else 1, np.int64),
+            "clist_val": np.zeros(2 * ncl, np.int64),
+            "plist": np.zeros(2 * n_node if l >= 1 else 0, np.int64),
+            "nlist_row": np.zeros(n_leaf + 1, np.int64), "nlist_val": np.zeros(2 * nnl, np.int64),
+        }
+        keys = ["c_node", "leaf2node", "idx_row", "idx_val", "ilist_row", "ilist_val", "clist_row", "clist_val",
+                "plist", "nlist_row", "nlist_val"]
+        _check(lib().sfmm_tree_level_lists(self.h, l, *[_i(out[k]) for k in keys]))
+        return out
+
+
+class FMM3D:
+    """FMM3D(partitioner, p, ncharge) (fmm.hpp:134-882) on the current HIP device.
+
+    rinv(Q, U[, gradient]) accumulates into U exactly as the reference; Q is
+    (n, ncharge) (or (n,) when ncharge == 1), U is (n, ncharge) float64.
+    rinv_potgrad(q, U) accumulates (phi, grad phi) of scalar charges, U (n, 4).
+    """
+
+    def __init__(self, partitioner: OctreePartitioner, p: int, ncharge: int = 3, precision: int = FP64):
+        if partitioner.h is None:
+            raise SfmmError("refine() the partitioner before building an FMM3D")
+        self.partitioner = partitioner  # the plan references the tree: keep it alive
+        self.p = int(p)
+        self.ncharge = int(ncharge)
+        desc = PlanDesc()
+        desc.p = self.p
+        desc.precision = int(precision)
+        desc.max_ncharge = max(1, min(4, self.ncharge))
+        h = ctypes.c_void_p()
+        _check(lib().sfmm_plan_create(partitioner.h, ctypes.byref(desc), ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.sfmm_plan_destroy(self.h)
+            self.h = None
+
+    @property
+    def n(self) -> int:
+        return self.partitioner.n
+
+    def _eval(self, kind: int, ncharge: int, Q, U, which: int):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        if not (isinstance(U, np.ndarray) and U.dtype == np.float64 and U.flags.c_contiguous):
+            raise TypeError("U must be a C-contiguous float64 numpy array (it is accumulated in place)")
+        _check(lib().sfmm_evaluate(self.h, kind, ncharge, _d(Q), _d(U), which))
+        return U
+
+    def rinv(self, Q, U, gradient: bool = False, which: int = BOTH):
+        """rinv<gradient>(Q, U): potentials sum Q_j/r, or (gradient) curl sum Q_j x R/|R|^3."""
+        if gradient:
+            return self._eval(CURL, 3, Q, U, which)
+        return self._eval(POTENTIAL, self.ncharge, Q, U, which)
+
+    def rinv_nonear(self, Q, U, gradient: bool = False):
+        return self.rinv(Q, U, gradient, FAR)
+
+    def N2N(self, Q, U, gradient: bool = False):
+        return self.rinv(Q, U, gradient, NEAR)
+
+    def rinv_potgrad(self, q, U, which: int = BOTH):
+        """Scalar charges: U[:, 0] += phi, U[:, 1:4] += grad phi."""
+        return self._eval(POTGRAD, 1, q, U, which)
+
+    # device-resident path (torch tensors or raw pointers) ------------------
+    def evaluate_device(self, kind: int, ncharge: int, q_ptr: int, u_ptr: int, which: int = BOTH):
+        _check(lib().sfmm_evaluate_device(self.h, kind, ncharge, ctypes.c_void_p(q_ptr), ctypes.c_void_p(u_ptr),
+                                          which))
+
+    def synchronize(self):
+        _check(lib().sfmm_synchronize(self.h))
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        _check(lib().sfmm_plan_stream(self.h, ctypes.byref(s)))
+        return s.value or 0
+
+    def stage_ms(self) -> dict:
+        ms = (ctypes.c_float * 9)()
+        _check(lib().sfmm_last_stage_ms(self.h, ms))
+        return dict(zip(STAGES, [float(x) for x in ms]))
+
+    def direct(self, kind: int, Q, targets, ncharge: int = 1):
+        """FP64 direct sum at original-index targets (all sources j != i)."""
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        targets = np.ascontiguousarray(targets, dtype=np.int64)
+        nout = {POTENTIAL: ncharge, CURL: 3, POTGRAD: 4}[kind]
+        out = np.zeros((len(targets), nout))
+        _check(lib().sfmm_direct(self.h, kind, ncharge, _d(Q), len(targets), _i(targets), _d(out)))
+        return out

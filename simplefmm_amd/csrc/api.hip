@@ -1,0 +1,544 @@
+// C-ABI of the FMM evaluator: host tree, device plan, evaluation orchestration.
+//
+// The plan mirrors FMM3D (icyspirit/SimpleFMM fmm.hpp:134-882): built once from
+// a partitioner, then evaluated any number of times with new charges.  The
+// level sweep of rinv_nonear (fmm.hpp:811-862) becomes one stream of kernels:
+// P2M, M2M per level bottom-up, M2L over all levels, L2L per level top-down,
+// L2P; rinv adds the near field (N2N, fmm.hpp:773-809).
+#include "../../include/sfmm.h"
+#include "kernels.hpp"
+#include "tables.hpp"
+#include "tree.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) return fail(SFMM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+struct sfmm_tree_s {
+    sfmm::Tree t;
+    std::vector<double> pos;  // copy of the caller's positions
+};
+
+struct sfmm_plan_s {
+    sfmm_tree_s* tree = nullptr;
+    int device = 0;
+    int p = 0, K = 0, precision = SFMM_FP64, max_ncharge = 1;
+    hipStream_t stream = nullptr;
+    sfmm::Tables tables;
+    sfmm::DevPlan d;
+    std::vector<void*> owned;
+    std::vector<int64_t> level_first;  // gnode of the first node of each level (size depth+2)
+    double* M = nullptr;
+    double* L = nullptr;
+    int nc_alloc = 0;
+    double* qbuf = nullptr;
+    double* obuf = nullptr;
+    int qdim_alloc = 0, nout_alloc = 0;
+    double* hQ = nullptr;  // device staging for host-array calls
+    double* hU = nullptr;
+    int64_t stage_q = 0, stage_u = 0;
+    hipEvent_t ev[9] = {};
+    bool timed = false;
+
+    ~sfmm_plan_s()
+    {
+        for (void* p : owned) (void)hipFree(p);
+        (void)hipFree(M);
+        (void)hipFree(L);
+        (void)hipFree(qbuf);
+        (void)hipFree(obuf);
+        (void)hipFree(hQ);
+        (void)hipFree(hU);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <class T>
+    int upload(const std::vector<T>& v, const T** out)
+    {
+        *out = nullptr;
+        if (v.empty()) return SFMM_OK;
+        void* p = nullptr;
+        if (hipMalloc(&p, v.size() * sizeof(T)) != hipSuccess) return fail(SFMM_ERR_ALLOC, "hipMalloc failed (plan tables)");
+        owned.push_back(p);
+        HIP_TRY(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        *out = static_cast<const T*>(p);
+        return SFMM_OK;
+    }
+};
+
+namespace {
+
+int mode_of(int kind, int ncharge, sfmm::Mode& mode, int& qdim, int& nc)
+{
+    switch (kind) {
+    case SFMM_POTENTIAL:
+        if (ncharge < 1 || ncharge > 4) return fail(SFMM_ERR_ARG, "ncharge must be in 1..4");
+        mode = sfmm::MODE_POT;
+        qdim = nc = ncharge;
+        return SFMM_OK;
+    case SFMM_CURL:
+        if (ncharge != 3) return fail(SFMM_ERR_ARG, "the curl field needs 3-vector charges (fmm.hpp:550)");
+        mode = sfmm::MODE_CURL;
+        qdim = nc = 3;
+        return SFMM_OK;
+    case SFMM_POTGRAD:
+        if (ncharge != 1) return fail(SFMM_ERR_ARG, "potential+gradient takes scalar charges");
+        mode = sfmm::MODE_POTGRAD;
+        qdim = 1;
+        nc = 4;
+        return SFMM_OK;
+    default: return fail(SFMM_ERR_ARG, "unknown evaluation kind");
+    }
+}
+
+int ensure_buffers(sfmm_plan_s* P, int nc, int qdim)
+{
+    const int64_t nnode = P->level_first.back();
+    const int64_t n = P->tree->t.n;
+    if (nc > P->nc_alloc) {
+        (void)hipFree(P->M);
+        (void)hipFree(P->L);
+        P->M = P->L = nullptr;
+        P->nc_alloc = 0;
+        const size_t bytes = sizeof(double) * static_cast<size_t>(nnode) * nc * P->K;
+        if (hipMalloc(&P->M, bytes) != hipSuccess || hipMalloc(&P->L, bytes) != hipSuccess)
+            return fail(SFMM_ERR_ALLOC, "hipMalloc failed (expansions)");
+        P->nc_alloc = nc;
+    }
+    if (qdim > P->qdim_alloc) {
+        (void)hipFree(P->qbuf);
+        P->qbuf = nullptr;
+        if (hipMalloc(&P->qbuf, sizeof(double) * n * qdim) != hipSuccess)
+            return fail(SFMM_ERR_ALLOC, "hipMalloc failed (charges)");
+        P->qdim_alloc = qdim;
+    }
+    if (nc > P->nout_alloc) {
+        (void)hipFree(P->obuf);
+        P->obuf = nullptr;
+        if (hipMalloc(&P->obuf, sizeof(double) * n * nc) != hipSuccess)
+            return fail(SFMM_ERR_ALLOC, "hipMalloc failed (outputs)");
+        P->nout_alloc = nc;
+    }
+    return SFMM_OK;
+}
+
+int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int which)
+{
+    sfmm::Mode mode;
+    int qdim, nc;
+    if (int rc = mode_of(kind, ncharge, mode, qdim, nc)) return rc;
+    if (which < 1 || which > 3) return fail(SFMM_ERR_ARG, "which must be SFMM_FAR, SFMM_NEAR or SFMM_BOTH");
+    if (mode == sfmm::MODE_POT && ncharge > P->max_ncharge)
+        return fail(SFMM_ERR_ARG, "ncharge exceeds the plan's max_ncharge");
+    HIP_TRY(hipSetDevice(P->device));
+    if (int rc = ensure_buffers(P, nc, qdim)) return rc;
+    const sfmm::DevPlan& d = P->d;
+    hipStream_t s = P->stream;
+    const int depth = P->tree->t.depth;
+    HIP_TRY(hipEventRecord(P->ev[0], s));
+    sfmm::launch_gather_charges(d, qdim, dQ, P->qbuf, s);
+    sfmm::launch_fill(P->obuf, static_cast<int64_t>(nc) * d.n, 0.0, s);
+    HIP_TRY(hipEventRecord(P->ev[1], s));
+    const bool far = (which & SFMM_FAR) && depth >= 2;  // fmm.hpp:816-820
+    if (far) sfmm::launch_p2m(d, mode, nc, P->qbuf, P->M, s);
+    HIP_TRY(hipEventRecord(P->ev[2], s));
+    if (far)
+        for (int l = depth - 1; l >= 2; --l)
+            sfmm::launch_m2m(d, {P->level_first[l], P->level_first[l + 1] - P->level_first[l]}, nc, P->M, s);
+    HIP_TRY(hipEventRecord(P->ev[3], s));
+    if (far) sfmm::launch_m2l(d, nc, P->M, P->L, s);
+    HIP_TRY(hipEventRecord(P->ev[4], s));
+    if (far)
+        for (int l = 3; l <= depth; ++l)
+            sfmm::launch_l2l(d, {P->level_first[l], P->level_first[l + 1] - P->level_first[l]}, nc, P->L, s);
+    HIP_TRY(hipEventRecord(P->ev[5], s));
+    if (far) sfmm::launch_l2p(d, nc, P->L, P->obuf, s);
+    HIP_TRY(hipEventRecord(P->ev[6], s));
+    if (which & SFMM_NEAR) sfmm::launch_p2p(d, mode, nc, P->qbuf, P->obuf, s);
+    HIP_TRY(hipEventRecord(P->ev[7], s));
+    sfmm::launch_scatter_outputs(d, nc, P->obuf, dU, s);
+    HIP_TRY(hipEventRecord(P->ev[8], s));
+    HIP_TRY(hipGetLastError());
+    P->timed = true;
+    return SFMM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sfmm_last_error(void) { return g_last_error.c_str(); }
+
+int sfmm_tree_create(const sfmm_tree_desc* desc, sfmm_tree* out)
+{
+    if (!desc || !out || !desc->positions) return fail(SFMM_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (desc->n <= 0) return fail(SFMM_ERR_ARG, "particle count must be positive");
+    if (desc->vertical_axis < 0 || desc->vertical_axis > 2) return fail(SFMM_ERR_ARG, "vertical_axis must be 0, 1 or 2");
+    auto T = std::make_unique<sfmm_tree_s>();
+    T->pos.assign(desc->positions, desc->positions + 3 * desc->n);
+    sfmm::BoxGeom box = desc->box_given
+                            ? sfmm::make_box_explicit(desc->corner, desc->width, desc->vertical_axis, desc->phideg)
+                            : sfmm::make_box(desc->n, T->pos.data(), desc->vertical_axis, desc->phideg);
+    int threads = desc->threads > 0 ? desc->threads : static_cast<int>(std::thread::hardware_concurrency());
+    if (threads <= 0) threads = 1;
+    std::string err = sfmm::build_tree(T->t, desc->n, T->pos.data(), box, desc->max_level, desc->max_per_leaf, threads);
+    if (!err.empty()) return fail(SFMM_ERR_ARG, err);
+    *out = T.release();
+    return SFMM_OK;
+}
+
+int sfmm_tree_destroy(sfmm_tree tree)
+{
+    delete tree;
+    return SFMM_OK;
+}
+
+int sfmm_tree_get_info(sfmm_tree tree, sfmm_tree_info* info)
+{
+    if (!tree || !info) return fail(SFMM_ERR_ARG, "null argument");
+    const sfmm::TreeStats st = sfmm::tree_stats(tree->t);
+    info->depth = tree->t.depth;
+    info->n_node = st.n_node;
+    info->n_leaf = st.n_leaf;
+    info->m2l_pairs = st.m2l_pairs;
+    info->near_pairs = st.near_pairs;
+    info->width = tree->t.box.width;
+    for (int d = 0; d < 3; ++d) info->corner[d] = tree->t.box.corner[d];
+    return SFMM_OK;
+}
+
+int sfmm_tree_level_counts(sfmm_tree tree, int level, int64_t counts[6])
+{
+    if (!tree || !counts) return fail(SFMM_ERR_ARG, "null argument");
+    if (level < 0 || level > tree->t.depth) return fail(SFMM_ERR_ARG, "level out of range");
+    sfmm::RefLists r;
+    sfmm::reference_lists(tree->t, level, r);
+    const auto& lv = tree->t.lv[level];
+    counts[0] = lv.n_node();
+    counts[1] = lv.n_leaf();
+    counts[2] = static_cast<int64_t>(r.idx_val.size());
+    counts[3] = static_cast<int64_t>(r.ilist_val.size() / 4);
+    counts[4] = static_cast<int64_t>(r.clist_val.size() / 2);
+    counts[5] = static_cast<int64_t>(r.nlist_val.size() / 2);
+    return SFMM_OK;
+}
+
+int sfmm_tree_level_lists(sfmm_tree tree, int level, int64_t* c_node, int64_t* leaf2node, int64_t* idx_row,
+                          int64_t* idx_val, int64_t* ilist_row, int64_t* ilist_val, int64_t* clist_row,
+                          int64_t* clist_val, int64_t* plist, int64_t* nlist_row, int64_t* nlist_val)
+{
+    if (!tree) return fail(SFMM_ERR_ARG, "null argument");
+    if (level < 0 || level > tree->t.depth) return fail(SFMM_ERR_ARG, "level out of range");
+    sfmm::RefLists r;
+    sfmm::reference_lists(tree->t, level, r);
+    auto put = [](int64_t* dst, const std::vector<int64_t>& v) {
+        if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(int64_t));
+    };
+    put(c_node, r.c_node);
+    put(leaf2node, r.leaf2node);
+    put(idx_row, r.idx_row);
+    put(idx_val, r.idx_val);
+    put(ilist_row, r.ilist_row);
+    put(ilist_val, r.ilist_val);
+    put(clist_row, r.clist_row);
+    put(clist_val, r.clist_val);
+    put(plist, r.plist);
+    put(nlist_row, r.nlist_row);
+    put(nlist_val, r.nlist_val);
+    return SFMM_OK;
+}
+
+int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
+{
+    if (!tree || !desc || !out) return fail(SFMM_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (desc->p < 1 || desc->p > 16) return fail(SFMM_ERR_ARG, "expansion order p must be in 1..16");
+    if (desc->precision != SFMM_FP64) return fail(SFMM_ERR_ARG, "only SFMM_FP64 is available in this build");
+    if (desc->self_row || desc->self_val) return fail(SFMM_ERR_ARG, "custom self exclusions are not available in this build");
+    if (desc->max_ncharge < 1 || desc->max_ncharge > 4) return fail(SFMM_ERR_ARG, "max_ncharge must be in 1..4");
+    auto P = std::make_unique<sfmm_plan_s>();
+    P->tree = tree;
+    P->p = desc->p;
+    P->K = (desc->p + 1) * (desc->p + 1);
+    P->precision = desc->precision;
+    P->max_ncharge = desc->max_ncharge;
+    HIP_TRY(hipGetDevice(&P->device));
+    HIP_TRY(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+    for (auto& e : P->ev) HIP_TRY(hipEventCreate(&e));
+    const sfmm::Tree& t = tree->t;
+    const int p = P->p, K = P->K;
+    sfmm::build_tables(p, P->tables);
+    const sfmm::Tables& tb = P->tables;
+
+    // ---- nodes, numbered level by level (gnode)
+    const int D = t.depth;
+    P->level_first.assign(D + 2, 0);
+    for (int l = 0; l <= D; ++l) P->level_first[l + 1] = P->level_first[l] + t.lv[l].n_node();
+    const int64_t nnode = P->level_first[D + 1];
+    std::vector<int8_t> octant(nnode), nchild(nnode);
+    std::vector<int32_t> parent(nnode), child0(nnode);
+    for (int l = 0; l <= D; ++l) {
+        const auto& lv = t.lv[l];
+        for (int32_t a = 0; a < lv.n_node(); ++a) {
+            const int64_t g = P->level_first[l] + a;
+            octant[g] = static_cast<int8_t>(lv.code[a] & 7);
+            nchild[g] = lv.nchild[a];
+            parent[g] = l > 0 ? static_cast<int32_t>(P->level_first[l - 1] + lv.parent[a]) : -1;
+            child0[g] = lv.child0[a] >= 0 ? static_cast<int32_t>(P->level_first[l + 1] + lv.child0[a]) : -1;
+        }
+    }
+    // ---- particles in sorted order
+    const int64_t n = t.n;
+    std::vector<double> px(n), py(n), pz(n), ux(n, 0.0), uy(n, 0.0), uz(n, 0.0);
+    std::vector<int32_t> iperm(n);
+    for (int64_t s = 0; s < n; ++s) {
+        const double* x = tree->pos.data() + 3 * static_cast<int64_t>(t.perm[s]);
+        px[s] = x[0];
+        py[s] = x[1];
+        pz[s] = x[2];
+        iperm[t.perm[s]] = static_cast<int32_t>(s);
+    }
+    std::vector<int32_t> fl_gnode, fl_level;
+    std::vector<int64_t> fl_begin, fl_end, lf_begin, lf_end;
+    for (size_t g = 0; g < t.leaf_node.size(); ++g) {
+        const int l = t.leaf_level[g];
+        const auto& lv = t.lv[l];
+        const int32_t a = t.leaf_node[g];
+        lf_begin.push_back(lv.pbegin[a]);
+        lf_end.push_back(lv.pend[a]);
+        if (l < 2) continue;
+        fl_gnode.push_back(static_cast<int32_t>(P->level_first[l] + a));
+        fl_level.push_back(l);
+        fl_begin.push_back(lv.pbegin[a]);
+        fl_end.push_back(lv.pend[a]);
+        double c[3];
+        t.box.centre(l, lv.code[a], c);
+        const double iw = std::ldexp(1.0, l) / t.box.width;
+        for (int64_t s = lv.pbegin[a]; s < lv.pend[a]; ++s) {
+            double r[3];
+            t.box.rotate(tree->pos.data() + 3 * static_cast<int64_t>(t.perm[s]), r);
+            ux[s] = (r[0] - c[0]) * iw;
+            uy[s] = (r[1] - c[1]) * iw;
+            uz[s] = (r[2] - c[2]) * iw;
+        }
+    }
+    // ---- M2L work: all nodes at levels >= 2
+    std::vector<int32_t> tgt, il_src;
+    std::vector<int16_t> il_off;
+    std::vector<int64_t> il_row{0};
+    for (int l = 2; l <= D; ++l) {
+        const auto& lv = t.lv[l];
+        for (int32_t a = 0; a < lv.n_node(); ++a) {
+            tgt.push_back(static_cast<int32_t>(P->level_first[l] + a));
+            for (int64_t k = lv.il_row[a]; k < lv.il_row[a + 1]; ++k) {
+                il_src.push_back(static_cast<int32_t>(P->level_first[l] + lv.il_src[k]));
+                il_off.push_back(lv.il_off[k]);
+            }
+            il_row.push_back(static_cast<int64_t>(il_src.size()));
+        }
+    }
+    std::vector<double> inv_width(D + 1);
+    for (int l = 0; l <= D; ++l) inv_width[l] = std::ldexp(1.0, l) / t.box.width;
+    // ---- tables
+    std::vector<double> m2mT(tb.m2m.size()), l2lT(tb.l2l.size());
+    for (int o = 0; o < 8; ++o)
+        for (int r = 0; r < K; ++r)
+            for (int c = 0; c < K; ++c) {
+                const size_t src = (static_cast<size_t>(o) * K + r) * K + c;
+                const size_t dst = (static_cast<size_t>(o) * K + c) * K + r;
+                m2mT[dst] = tb.m2m[src];
+                l2lT[dst] = tb.l2l[src];
+            }
+    std::vector<int32_t> om_rows(K);
+    for (int r = 0; r < K; ++r) {
+        int nn = 0;
+        while ((nn + 1) * (nn + 1) <= r) ++nn;
+        const int qq = r - nn * nn;
+        const int m = (qq + 1) / 2;
+        const int part = (qq == 0 || (qq & 1)) ? 0 : 1;
+        om_rows[tb.om[r]] = m | (nn << 8) | (part << 16);
+    }
+    std::vector<double> rec(3 * 21 * 21, 0.0);
+    for (int nn = 0; nn <= 20; ++nn)
+        for (int m = 0; m < nn; ++m) {
+            const double den = std::sqrt(static_cast<double>(nn * nn - m * m));
+            rec[nn * 21 + m] = (2 * nn - 1) / den;
+            rec[21 * 21 + nn * 21 + m] = std::sqrt(static_cast<double>((nn - 1) * (nn - 1) - m * m)) / den;
+        }
+    for (int m = 1; m <= 20; ++m) rec[2 * 21 * 21 + m] = std::sqrt((2.0 * m - 1) / (2.0 * m));
+
+    sfmm::DevPlan& d = P->d;
+    d.p = p;
+    d.K = K;
+    d.depth = D;
+    d.n = n;
+    d.n_far_leaf = static_cast<int64_t>(fl_gnode.size());
+    d.n_leaf = static_cast<int64_t>(lf_begin.size());
+    d.n_target = static_cast<int64_t>(tgt.size());
+    d.rot_size = tb.rot_size;
+    d.coax_size = tb.coax_size;
+    int rc = SFMM_OK;
+#define UP(vec, field) \
+    if ((rc = P->upload(vec, &d.field)) != SFMM_OK) return rc
+    UP(px, px); UP(py, py); UP(pz, pz); UP(ux, ux); UP(uy, uy); UP(uz, uz);
+    UP(t.perm, perm); UP(iperm, iperm);
+    UP(fl_gnode, fl_gnode); UP(fl_level, fl_level); UP(fl_begin, fl_begin); UP(fl_end, fl_end);
+    UP(lf_begin, lf_begin); UP(lf_end, lf_end);
+    UP(t.near_row, near_row); UP(t.near_begin, near_begin); UP(t.near_end, near_end);
+    UP(octant, octant); UP(parent, parent); UP(child0, child0); UP(nchild, nchild);
+    UP(tgt, tgt_gnode); UP(il_row, il_row); UP(il_src, il_src); UP(il_off, il_off);
+    UP(inv_width, inv_width);
+    UP(m2mT, m2mT); UP(l2lT, l2lT); UP(tb.rot, rot); UP(tb.coax, coax); UP(tb.off_cs, off_cs);
+    UP(tb.off_class, off_class); UP(tb.off_d2, off_d2); UP(tb.om, om); UP(om_rows, om_rows);
+    UP(tb.rot_off, rot_off); UP(tb.coax_off, coax_off); UP(rec, rec);
+#undef UP
+    *out = P.release();
+    return SFMM_OK;
+}
+
+int sfmm_plan_destroy(sfmm_plan plan)
+{
+    if (plan) {
+        (void)hipSetDevice(plan->device);
+        (void)hipStreamSynchronize(plan->stream);
+    }
+    delete plan;
+    return SFMM_OK;
+}
+
+int sfmm_plan_stream(sfmm_plan plan, void** stream)
+{
+    if (!plan || !stream) return fail(SFMM_ERR_ARG, "null argument");
+    *stream = plan->stream;
+    return SFMM_OK;
+}
+
+int sfmm_synchronize(sfmm_plan plan)
+{
+    if (!plan) return fail(SFMM_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(plan->device));
+    HIP_TRY(hipStreamSynchronize(plan->stream));
+    return SFMM_OK;
+}
+
+int sfmm_evaluate_device(sfmm_plan plan, int kind, int ncharge, const double* dQ, double* dU, int which)
+{
+    if (!plan || !dQ || !dU) return fail(SFMM_ERR_ARG, "null argument");
+    return run(plan, kind, ncharge, dQ, dU, which);
+}
+
+int sfmm_evaluate(sfmm_plan plan, int kind, int ncharge, const double* Q, double* U, int which)
+{
+    if (!plan || !Q || !U) return fail(SFMM_ERR_ARG, "null argument");
+    sfmm::Mode mode;
+    int qdim, nc;
+    if (int rc = mode_of(kind, ncharge, mode, qdim, nc)) return rc;
+    HIP_TRY(hipSetDevice(plan->device));
+    const int64_t n = plan->tree->t.n;
+    if (qdim * n > plan->stage_q) {
+        (void)hipFree(plan->hQ);
+        plan->hQ = nullptr;
+        if (hipMalloc(&plan->hQ, sizeof(double) * qdim * n) != hipSuccess) return fail(SFMM_ERR_ALLOC, "hipMalloc failed (staging)");
+        plan->stage_q = qdim * n;
+    }
+    if (nc * n > plan->stage_u) {
+        (void)hipFree(plan->hU);
+        plan->hU = nullptr;
+        if (hipMalloc(&plan->hU, sizeof(double) * nc * n) != hipSuccess) return fail(SFMM_ERR_ALLOC, "hipMalloc failed (staging)");
+        plan->stage_u = nc * n;
+    }
+    HIP_TRY(hipMemcpyAsync(plan->hQ, Q, sizeof(double) * qdim * n, hipMemcpyHostToDevice, plan->stream));
+    sfmm::launch_fill(plan->hU, nc * n, 0.0, plan->stream);
+    if (int rc = run(plan, kind, ncharge, plan->hQ, plan->hU, which)) return rc;
+    std::vector<double> tmp(static_cast<size_t>(nc) * n);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), plan->hU, sizeof(double) * nc * n, hipMemcpyDeviceToHost, plan->stream));
+    HIP_TRY(hipStreamSynchronize(plan->stream));
+    for (size_t i = 0; i < tmp.size(); ++i) U[i] += tmp[i];
+    return SFMM_OK;
+}
+
+int sfmm_last_stage_ms(sfmm_plan plan, float ms[9])
+{
+    if (!plan || !ms) return fail(SFMM_ERR_ARG, "null argument");
+    if (!plan->timed) return fail(SFMM_ERR_STATE, "no evaluation has run yet");
+    HIP_TRY(hipSetDevice(plan->device));
+    HIP_TRY(hipEventSynchronize(plan->ev[8]));
+    for (int k = 0; k < 8; ++k) HIP_TRY(hipEventElapsedTime(&ms[k], plan->ev[k], plan->ev[k + 1]));
+    HIP_TRY(hipEventElapsedTime(&ms[8], plan->ev[0], plan->ev[8]));
+    return SFMM_OK;
+}
+
+int sfmm_direct(sfmm_plan plan, int kind, int ncharge, const double* Q, int64_t ntarget, const int64_t* targets,
+                double* out)
+{
+    if (!plan || !Q || !targets || !out) return fail(SFMM_ERR_ARG, "null argument");
+    sfmm::Mode mode;
+    int qdim, nc;
+    if (int rc = mode_of(kind, ncharge, mode, qdim, nc)) return rc;
+    const int64_t n = plan->tree->t.n;
+    for (int64_t k = 0; k < ntarget; ++k)
+        if (targets[k] < 0 || targets[k] >= n) return fail(SFMM_ERR_ARG, "target index out of range");
+    if (ntarget <= 0) return SFMM_OK;
+    HIP_TRY(hipSetDevice(plan->device));
+    if (int rc = ensure_buffers(plan, nc, qdim)) return rc;
+    const int64_t nblk = (ntarget + 63) / 64;
+    int nsplit = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(512, 8192 / nblk)));
+    nsplit = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(nsplit, n / 64 + 1)));
+    double *dQ = nullptr, *dout = nullptr, *part = nullptr;
+    int64_t* dt = nullptr;
+    auto cleanup = [&]() {
+        (void)hipFree(dQ);
+        (void)hipFree(dout);
+        (void)hipFree(part);
+        (void)hipFree(dt);
+    };
+    if (hipMalloc(&dQ, sizeof(double) * qdim * n) != hipSuccess ||
+        hipMalloc(&dout, sizeof(double) * nc * ntarget) != hipSuccess ||
+        hipMalloc(&part, sizeof(double) * sfmm::direct_scratch_size(ntarget, nc, nsplit)) != hipSuccess ||
+        hipMalloc(&dt, sizeof(int64_t) * ntarget) != hipSuccess) {
+        cleanup();
+        return fail(SFMM_ERR_ALLOC, "hipMalloc failed (direct sum)");
+    }
+    hipStream_t s = plan->stream;
+    hipError_t e = hipMemcpyAsync(dQ, Q, sizeof(double) * qdim * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dt, targets, sizeof(int64_t) * ntarget, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        sfmm::launch_gather_charges(plan->d, qdim, dQ, plan->qbuf, s);
+        sfmm::launch_direct(plan->d, mode, nc, plan->qbuf, ntarget, dt, dout, part, nsplit, s);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(double) * nc * ntarget, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    cleanup();
+    if (e != hipSuccess) return fail(SFMM_ERR_HIP, std::string("direct sum: ") + hipGetErrorString(e));
+    return SFMM_OK;
+}
+
+}  // extern "C"

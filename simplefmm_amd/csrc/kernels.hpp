@@ -1,0 +1,76 @@
+// HIP kernels of the FMM evaluation (gfx950).  Launch wrappers only; the
+// kernels live in kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sfmm {
+
+// What a charge is and what is written per particle.
+enum Mode : int {
+    MODE_POT = 0,      // nc charge components -> nc potentials (rinv)
+    MODE_CURL = 1,     // 3-vector charge -> curl field Q x R/|R|^3 (rinv<true>)
+    MODE_POTGRAD = 2,  // scalar charge -> potential and its gradient (4 outputs)
+};
+
+// Device view of everything a launch needs.  All pointers are device memory.
+struct DevPlan {
+    int p = 0, K = 0, depth = 0;
+    int64_t n = 0;
+    // particles in sorted ("slot") order
+    const double *px = nullptr, *py = nullptr, *pz = nullptr;  // raw positions
+    const double *ux = nullptr, *uy = nullptr, *uz = nullptr;  // (rotated - leaf centre) / leaf width
+    const int32_t* perm = nullptr;                             // slot -> original index
+    const int32_t* iperm = nullptr;                            // original index -> slot
+    // leaves with far field (level >= 2)
+    int64_t n_far_leaf = 0;
+    const int32_t *fl_gnode = nullptr, *fl_level = nullptr;
+    const int64_t *fl_begin = nullptr, *fl_end = nullptr;
+    // all leaves, near field
+    int64_t n_leaf = 0;
+    const int64_t *lf_begin = nullptr, *lf_end = nullptr;
+    const int64_t *near_row = nullptr, *near_begin = nullptr, *near_end = nullptr;
+    // nodes
+    const int8_t* octant = nullptr;     // code & 7 per gnode
+    const int32_t* parent = nullptr;    // parent gnode (-1 at level 0)
+    const int32_t* child0 = nullptr;    // first child gnode (-1 for leaves)
+    const int8_t* nchild = nullptr;
+    // M2L work: targets (gnodes at levels >= 2) and their interaction lists
+    int64_t n_target = 0;
+    const int32_t* tgt_gnode = nullptr;
+    const int64_t* il_row = nullptr;
+    const int32_t* il_src = nullptr;   // source gnode
+    const int16_t* il_off = nullptr;   // offset id
+    const double* inv_width = nullptr;  // per level 1/w_l
+    // tables
+    const double *m2mT = nullptr, *l2lT = nullptr;  // [oct][col][row]
+    const double *rot = nullptr, *coax = nullptr, *off_cs = nullptr;
+    const int32_t *off_class = nullptr, *off_d2 = nullptr;
+    const int32_t *om = nullptr, *om_rows = nullptr, *rot_off = nullptr, *coax_off = nullptr;
+    int rot_size = 0, coax_size = 0;
+    const double* rec = nullptr;  // recurrence coefficients, see kernels.hip
+};
+
+struct LevelSpan {
+    int64_t first = 0, count = 0;  // gnode range of one level
+};
+
+// All launches are asynchronous on `stream`.
+void launch_gather_charges(const DevPlan& d, int qdim, const double* Q, double* q, hipStream_t s);
+void launch_scatter_outputs(const DevPlan& d, int nout, const double* out, double* U, hipStream_t s);
+void launch_p2m(const DevPlan& d, Mode mode, int nc, const double* q, double* M, hipStream_t s);
+void launch_m2m(const DevPlan& d, LevelSpan parents, int nc, double* M, hipStream_t s);
+void launch_m2l(const DevPlan& d, int nc, const double* M, double* L, hipStream_t s);
+void launch_l2l(const DevPlan& d, LevelSpan children, int nc, double* L, hipStream_t s);
+void launch_l2p(const DevPlan& d, int nc, const double* L, double* out, hipStream_t s);
+void launch_p2p(const DevPlan& d, Mode mode, int nc, const double* q, double* out, hipStream_t s);
+void launch_direct(const DevPlan& d, Mode mode, int nc, const double* q, int64_t ntarget, const int64_t* targets,
+                   double* out, double* partial, int nsplit, hipStream_t s);
+void launch_fill(double* x, int64_t count, double v, hipStream_t s);
+
+// Number of double slots the direct sum needs as scratch for `ntarget` targets.
+int64_t direct_scratch_size(int64_t ntarget, int nout, int nsplit);
+
+}  // namespace sfmm

@@ -1,0 +1,259 @@
+// Operator tables for the FMM plan (see tables.hpp for the storage convention).
+//
+// The operators are those of the reference (icyspirit/SimpleFMM fmm.hpp):
+//   M2M  fmm.hpp:558-576   L2L  fmm.hpp:714-733
+//   M2L  rotate - coaxial translate - rotate, fmm.hpp:620-688, tables :339-396
+// restated here in the level-scaled real form.
+#include "tables.hpp"
+
+#include <cmath>
+#include <complex>
+
+namespace sfmm {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+using cd = std::complex<double>;
+
+inline int iabs(int x) { return x < 0 ? -x : x; }
+inline double neg1(int e) { return (e % 2 == 0) ? 1.0 : -1.0; }
+inline int fkm(int k, int m) { return (iabs(k) - iabs(m) - iabs(k - m)) >> 1; }  // fmm.hpp:128-131
+inline int ci(int n, int m) { return n * (n + 1) + m; }                           // complex index
+
+double Acoef(int n, int m) { return 1 / std::sqrt(std::tgamma(n - m + 1) * std::tgamma(n + m + 1)); }
+
+double Zval(int n, int m, double theta)  // fmm.hpp:107-111
+{
+    return std::sqrt(4 * kPi / (2 * n + 1)) * std::sph_legendre(n, iabs(m), theta);
+}
+
+long double fact(int k)
+{
+    long double f = 1;
+    for (int i = 2; i <= k; ++i) f *= i;
+    return f;
+}
+
+// child octant geometry (fmm.hpp:444-478): azimuth of (child - parent) and of
+// (parent - child); the polar angle is acos(1/sqrt 3) or its supplement.
+double phi_child(int c)
+{
+    static const double v[4] = {-0.75 * kPi, -0.25 * kPi, 0.75 * kPi, 0.25 * kPi};
+    return v[c % 4];
+}
+double phi_parent(int c)
+{
+    static const double v[4] = {0.25 * kPi, 0.75 * kPi, -0.25 * kPi, -0.75 * kPi};
+    return v[c % 4];
+}
+
+}  // namespace
+
+void wigner_rotation(int n, long double theta, std::vector<double>& R)
+{
+    // Wigner's explicit sum for d^n_{ab}(beta) at beta = -theta
+    const int w = 2 * n + 1;
+    R.assign(static_cast<size_t>(w) * w, 0.0);
+    const long double c = std::cos(-theta / 2), s = std::sin(-theta / 2);
+    std::vector<long double> cp(2 * n + 1), sp(2 * n + 1);
+    cp[0] = sp[0] = 1;
+    for (int e = 1; e <= 2 * n; ++e) {
+        cp[e] = cp[e - 1] * c;
+        sp[e] = sp[e - 1] * s;
+    }
+    for (int a = -n; a <= n; ++a)
+        for (int b = -n; b <= n; ++b) {
+            const long double pref = std::sqrt(fact(n + a) * fact(n - a) * fact(n + b) * fact(n - b));
+            long double sum = 0;
+            for (int t = std::max(0, b - a); t <= std::min(n + b, n - a); ++t) {
+                const long double term =
+                    pref / (fact(n + b - t) * fact(t) * fact(a - b + t) * fact(n - a - t)) * cp[2 * n + b - a - 2 * t] *
+                    sp[a - b + 2 * t];
+                sum += ((a - b + t) % 2 == 0) ? term : -term;
+            }
+            const double sa = a < 0 ? neg1(a) : 1.0, sb = b < 0 ? neg1(b) : 1.0;
+            R[static_cast<size_t>(a + n) * w + (b + n)] = sa * sb * static_cast<double>(sum);
+        }
+}
+
+void complex_to_real_form(int p, const std::vector<double>& Tre, const std::vector<double>& Tim,
+                          std::vector<double>& out)
+{
+    const int K = (p + 1) * (p + 1);
+    out.assign(static_cast<size_t>(K) * K, 0.0);
+    auto T = [&](int j, int k, int n, int m) {
+        const size_t idx = static_cast<size_t>(ci(j, k)) * K + ci(n, m);
+        return cd(Tre[idx], Tim[idx]);
+    };
+    for (int j = 0; j <= p; ++j)
+        for (int k = 0; k <= j; ++k) {
+            const size_t ro = static_cast<size_t>(rf_cos(j, k)) * K;
+            const size_t rs = static_cast<size_t>(k > 0 ? rf_sin(j, k) : 0) * K;
+            for (int n = 0; n <= p; ++n) {
+                const cd t0 = T(j, k, n, 0);
+                out[ro + rf_cos(n, 0)] = t0.real();
+                if (k > 0) out[rs + rf_cos(n, 0)] = t0.imag();
+                for (int m = 1; m <= n; ++m) {
+                    const cd S = T(j, k, n, m) + T(j, k, n, -m);
+                    const cd D = T(j, k, n, m) - T(j, k, n, -m);
+                    out[ro + rf_cos(n, m)] = S.real();
+                    out[ro + rf_sin(n, m)] = -D.imag();
+                    if (k > 0) {
+                        out[rs + rf_cos(n, m)] = S.imag();
+                        out[rs + rf_sin(n, m)] = D.real();
+                    }
+                }
+            }
+        }
+}
+
+void build_tables(int p, Tables& t)
+{
+    t = Tables{};
+    t.p = p;
+    const int K = (p + 1) * (p + 1);
+    t.K = K;
+    const double th_diag = std::acos(1 / std::sqrt(3.0));
+    std::vector<double> zdiag(K);
+    for (int n = 0; n <= p; ++n)
+        for (int m = 0; m <= n; ++m) zdiag[ci(n, m)] = Zval(n, m, th_diag);
+
+    // ---- M2M, scaled: M~p(j,k) = sum (-1)^fkm(k,m) A(n,m)A(j-n,k-m)/A(j,k)
+    //      (sqrt .75)^n 2^-j Zc(n,-m) e^{-i m phi_c} M~c(j-n,k-m)
+    t.m2m.resize(static_cast<size_t>(8) * K * K);
+    t.l2l.resize(static_cast<size_t>(8) * K * K);
+    for (int cc = 0; cc < 8; ++cc) {
+        std::vector<double> Tre(static_cast<size_t>(K) * K, 0.0), Tim(static_cast<size_t>(K) * K, 0.0);
+        for (int j = 0; j <= p; ++j)
+            for (int k = -j; k <= j; ++k)
+                for (int n = 0; n <= j; ++n)
+                    for (int m = std::max(-n, k - (j - n)); m <= std::min(n, k + (j - n)); ++m) {
+                        const double zc = (((cc >> 2) % 2 == 0) ? neg1(n - iabs(m)) : 1.0) * zdiag[ci(n, iabs(m))];
+                        const double v = neg1(fkm(k, m)) * Acoef(n, m) * Acoef(j - n, k - m) / Acoef(j, k) *
+                                         std::pow(std::sqrt(0.75), n) * std::ldexp(1.0, -j) * zc;
+                        const cd e = std::polar(1.0, -m * phi_child(cc));
+                        const size_t idx = static_cast<size_t>(ci(j, k)) * K + ci(j - n, k - m);
+                        Tre[idx] += v * e.real();
+                        Tim[idx] += v * e.imag();
+                    }
+        std::vector<double> R;
+        complex_to_real_form(p, Tre, Tim, R);
+        std::copy(R.begin(), R.end(), t.m2m.begin() + static_cast<size_t>(cc) * K * K);
+
+        // ---- L2L, scaled: L~c(j,k) = sum_{n>=j} (-1)^(fkm(m,m-k)+n+j) A(n-j,m-k)A(j,k)/A(n,m)
+        //      (sqrt .75)^(n-j) 2^-(n+1) Zp(n-j,m-k) e^{i(m-k) phi_p} L~p(n,m)
+        std::fill(Tre.begin(), Tre.end(), 0.0);
+        std::fill(Tim.begin(), Tim.end(), 0.0);
+        for (int j = 0; j <= p; ++j)
+            for (int k = -j; k <= j; ++k)
+                for (int n = j; n <= p; ++n)
+                    for (int m = std::max(-n, k - (n - j)); m <= std::min(n, k + (n - j)); ++m) {
+                        const int nn = n - j, mm = m - k;
+                        const double zp = (((cc >> 2) % 2 == 0) ? 1.0 : neg1(nn - iabs(mm))) * zdiag[ci(nn, iabs(mm))];
+                        const double v = neg1(fkm(m, m - k) + n + j) * Acoef(nn, mm) * Acoef(j, k) / Acoef(n, m) *
+                                         std::pow(std::sqrt(0.75), nn) * std::ldexp(1.0, -(n + 1)) * zp;
+                        const cd e = std::polar(1.0, mm * phi_parent(cc));
+                        const size_t idx = static_cast<size_t>(ci(j, k)) * K + ci(n, m);
+                        Tre[idx] += v * e.real();
+                        Tim[idx] += v * e.imag();
+                    }
+        complex_to_real_form(p, Tre, Tim, R);
+        std::copy(R.begin(), R.end(), t.l2l.begin() + static_cast<size_t>(cc) * K * K);
+    }
+
+    // ---- rotations for the 70 polar classes (t = di^2+dj^2 in tvals, dk in [-3,3])
+    static const int tvals[10] = {0, 1, 2, 4, 5, 8, 9, 10, 13, 18};
+    t.rot_off.resize(p + 2);
+    t.rot_off[0] = 0;
+    for (int n = 0; n <= p; ++n) t.rot_off[n + 1] = t.rot_off[n] + (n + 1) * (n + 1) + n * n;
+    t.rot_size = t.rot_off[p + 1];
+    t.rot.assign(static_cast<size_t>(70) * 2 * t.rot_size, 0.0);
+    for (int ti = 0; ti < 10; ++ti)
+        for (int dk = -3; dk <= 3; ++dk) {
+            if (tvals[ti] == 0 && dk == 0) continue;
+            const long double theta = std::acos(dk / std::sqrt(static_cast<long double>(tvals[ti] + dk * dk)));
+            double* fwd = t.rot.data() + static_cast<size_t>(7 * ti + dk + 3) * 2 * t.rot_size;
+            double* bwd = fwd + t.rot_size;
+            std::vector<double> R;
+            for (int n = 0; n <= p; ++n) {
+                wigner_rotation(n, theta, R);
+                const int w = 2 * n + 1;
+                auto r = [&](int a, int b) { return R[static_cast<size_t>(a + n) * w + (b + n)]; };
+                double* fc = fwd + t.rot_off[n];
+                double* fs = fc + (n + 1) * (n + 1);
+                double* bc = bwd + t.rot_off[n];
+                double* bs = bc + (n + 1) * (n + 1);
+                for (int a = 0; a <= n; ++a) {
+                    fc[a * (n + 1) + 0] = r(a, 0);
+                    bc[a * (n + 1) + 0] = r(0, a);
+                    for (int b = 1; b <= n; ++b) {
+                        fc[a * (n + 1) + b] = r(a, b) + r(a, -b);
+                        bc[a * (n + 1) + b] = r(b, a) + r(-b, a);
+                    }
+                }
+                for (int a = 1; a <= n; ++a)
+                    for (int b = 1; b <= n; ++b) {
+                        fs[(a - 1) * n + (b - 1)] = r(a, b) - r(a, -b);
+                        bs[(a - 1) * n + (b - 1)] = r(b, a) - r(-b, a);
+                    }
+            }
+        }
+
+    // ---- coaxial translation blocks for |d|^2 = 0..27
+    t.coax_off.resize(p + 2);
+    t.coax_off[0] = 0;
+    for (int k = 0; k <= p; ++k) t.coax_off[k + 1] = t.coax_off[k] + (p + 1 - k) * (p + 1 - k);
+    t.coax_size = t.coax_off[p + 1];
+    t.coax.assign(static_cast<size_t>(28) * t.coax_size, 0.0);
+    for (int d2 = 1; d2 <= 27; ++d2) {
+        const double d = std::sqrt(static_cast<double>(d2));
+        double* C = t.coax.data() + static_cast<size_t>(d2) * t.coax_size;
+        for (int k = 0; k <= p; ++k) {
+            const int w = p + 1 - k;
+            for (int j = k; j <= p; ++j)
+                for (int n = k; n <= p; ++n)
+                    C[t.coax_off[k] + (j - k) * w + (n - k)] =
+                        neg1(n + k) * Acoef(n, k) * Acoef(j, k) / Acoef(j + n, 0) * std::pow(d, -(j + n + 1));
+        }
+    }
+
+    // ---- per offset: polar class, |d|^2, azimuth table
+    t.off_class.assign(343, -1);
+    t.off_d2.assign(343, 0);
+    t.off_cs.assign(static_cast<size_t>(343) * (p + 1) * 2, 0.0);
+    for (int dk = -3; dk <= 3; ++dk)
+        for (int dj = -3; dj <= 3; ++dj)
+            for (int di = -3; di <= 3; ++di) {
+                const int id = ((dk + 3) * 7 + (dj + 3)) * 7 + (di + 3);
+                const int tt = di * di + dj * dj;
+                int ti = -1;
+                for (int q = 0; q < 10; ++q)
+                    if (tvals[q] == tt) ti = q;
+                if (ti < 0 || (tt == 0 && dk == 0)) continue;
+                t.off_class[id] = 7 * ti + dk + 3;
+                t.off_d2[id] = tt + dk * dk;
+                const double ph = std::atan2(static_cast<double>(dj), static_cast<double>(di));
+                for (int m = 0; m <= p; ++m) {
+                    const cd e = std::polar(1.0, m * ph);
+                    t.off_cs[(static_cast<size_t>(id) * (p + 1) + m) * 2] = e.real();
+                    t.off_cs[(static_cast<size_t>(id) * (p + 1) + m) * 2 + 1] = e.imag();
+                }
+            }
+
+    // ---- order-major positions for the coaxial stage
+    t.om.assign(K, 0);
+    std::vector<int> ooff(p + 2, 0);
+    ooff[0] = 0;
+    ooff[1] = p + 1;
+    for (int k = 1; k <= p; ++k) ooff[k + 1] = ooff[k] + 2 * (p + 1 - k);
+    for (int n = 0; n <= p; ++n) {
+        t.om[rf_cos(n, 0)] = n;
+        for (int k = 1; k <= n; ++k) {
+            t.om[rf_cos(n, k)] = ooff[k] + (n - k);
+            t.om[rf_sin(n, k)] = ooff[k] + (p + 1 - k) + (n - k);
+        }
+    }
+}
+
+}  // namespace sfmm

@@ -1,0 +1,36 @@
+"""The C-ABI library loads and exports every entry point include/sfmm.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+from simplefmm_amd import api
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "sfmm.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sfmm_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert "sfmm_plan_create" in names and "sfmm_evaluate_device" in names
+    assert len(names) >= 14
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(api.LIB_PATH)
+    for name in _declared():
+        assert hasattr(L, name), name
+
+
+def test_binding_covers_header():
+    assert set(_declared()) <= set(api.SIGNATURES)
+
+
+def test_last_error_is_a_string():
+    L = api.lib()
+    assert isinstance(L.sfmm_last_error(), bytes)

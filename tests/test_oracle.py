@@ -1,0 +1,164 @@
+"""Pins the CPU oracle (oracle/) before it is trusted as the checker.
+
+* against the reference's own outputs (golden fixtures written by
+  tests/golden/make_golden.py from the reference FMM) at 1e-12 relative L2;
+* against what the mathematics fixes: closed forms of Z and A, orthogonality
+  and composition of the Wigner rotations, finite differences of the harmonic
+  gradients, the dense M2L against rotate-translate-rotate, additivity of the
+  near and far parts, and convergence to the direct sum as p grows.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from simplefmm_amd import workloads as W
+from tests.helpers import golden_cases, load_golden, rel_l2
+
+SMALL_POTGRAD = [c for c in golden_cases("potgrad") if load_golden(c)[0]["n"] <= 20000]
+
+
+@pytest.mark.parametrize("case", SMALL_POTGRAD)
+def test_oracle_matches_reference_potgrad(case):
+    meta, g, pos, q = load_golden(case)
+    t = O.Tree(pos, meta["max_level"], meta["cap"])
+    assert t.depth == meta["depth"]
+    idx = g["idx"]
+    for which, suffix in ((O.BOTH, "full"), (O.FAR, "far"), (O.NEAR, "near")):
+        if f"phi_{suffix}" not in g:
+            continue
+        out = t.fmm(meta["p"], q, O.POTGRAD, which)
+        assert rel_l2(out[idx, 0], g[f"phi_{suffix}"]) < 1e-12
+        assert rel_l2(out[idx, 1:], g[f"grad_{suffix}"]) < 1e-12
+
+
+def test_oracle_matches_reference_vector():
+    meta, g, pos, Q = load_golden("vector_5k_p8")
+    t = O.Tree(pos, meta["max_level"], meta["cap"])
+    idx = g["idx"]
+    for which, suffix in ((O.BOTH, "full"), (O.FAR, "far"), (O.NEAR, "near")):
+        pot = t.fmm(meta["p"], Q, O.VECPOT, which)
+        curl = t.fmm(meta["p"], Q, O.CURL, which)
+        assert rel_l2(pot[idx], g[f"pot_{suffix}"]) < 1e-12
+        assert rel_l2(curl[idx], g[f"curl_{suffix}"]) < 1e-12
+
+
+def test_curl_identity_in_fixtures():
+    # the fixture's gradient comes from two curl runs; dphi/dz appears in both
+    for case in golden_cases("potgrad"):
+        _, g, _, _ = load_golden(case)
+        if "gz_identity_maxdiff" in g:
+            assert g["gz_identity_maxdiff"][0] < 1e-9
+
+
+def test_Z_closed_forms():
+    for th in (0.3, 1.1, 2.5):
+        for n in range(0, 12):
+            assert abs(O.Z(n, 0, 0.0) - 1.0) < 1e-14
+            zn = (-1) ** n * math.sqrt(math.factorial(2 * n)) / (2 ** n * math.factorial(n)) * math.sin(th) ** n
+            assert abs(O.Z(n, n, th) - zn) < 1e-13 * max(1.0, abs(zn))
+            assert O.Z(n, -n, th) == O.Z(n, n, th)
+        assert abs(O.Z(1, 1, th) + math.sin(th) / math.sqrt(2)) < 1e-15
+        assert abs(O.Z(2, 0, th) - 0.5 * (3 * math.cos(th) ** 2 - 1)) < 1e-14
+
+
+def test_A_closed_form():
+    for n in range(0, 17):
+        for m in range(-n, n + 1):
+            ref = 1.0 / math.sqrt(math.factorial(n - m) * math.factorial(n + m))
+            assert abs(O.A(n, m) - ref) < 1e-14 * ref
+
+
+def _R(n, th):
+    return np.array([[O.wigner_R(n, a, b, th) for b in range(-n, n + 1)] for a in range(-n, n + 1)])
+
+
+def test_wigner_rotation_properties():
+    for n in (1, 4, 9, 16):
+        for th in (0.0, 0.7, 2.2, math.pi):
+            R = _R(n, th)
+            assert np.max(np.abs(R @ R.T - np.eye(2 * n + 1))) < 1e-13
+        assert np.max(np.abs(_R(n, 0.0) - np.eye(2 * n + 1))) < 1e-15
+        assert np.max(np.abs(_R(n, 0.4) @ _R(n, 0.9) - _R(n, 1.3))) < 1e-13
+    # degree 1 closed form: d^1_{00}(-t) = cos t, d^1_{11}(-t) = (1+cos t)/2
+    t = 0.8
+    R1 = _R(1, t)
+    assert abs(R1[1, 1] - math.cos(t)) < 1e-15
+    assert abs(R1[2, 2] - (1 + math.cos(t)) / 2) < 1e-15
+
+
+def test_harmonic_gradient_finite_difference():
+    p = 8
+    rng = np.random.default_rng(4)
+    for _ in range(5):
+        x = rng.normal(size=3) * 0.4
+        h = 1e-6
+        for d in range(3):
+            e = np.zeros(3)
+            e[d] = h
+            fd = (O.harmonic(p, *(x + e)) - O.harmonic(p, *(x - e))) / (2 * h)
+            g = O.harmonic(p, *x, grad=d + 1)
+            assert np.max(np.abs(fd - g)) < 1e-7
+
+
+def test_dense_m2l_equals_rotation_m2l():
+    pos, q = W.uniform_cube(3000, seed=21)
+    t = O.Tree(pos, 20, 32)
+    a = t.fmm(6, q, O.POTGRAD, O.FAR)
+    b = t.fmm(6, q, O.POTGRAD, O.FAR, dense=True)
+    assert rel_l2(a, b) < 1e-13
+
+
+def test_near_plus_far_is_both():
+    pos, q = W.clusters(4000, seed=22)
+    t = O.Tree(pos, 20, 24)
+    both = t.fmm(5, q, O.POTGRAD, O.BOTH)
+    parts = t.fmm(5, q, O.POTGRAD, O.FAR) + t.fmm(5, q, O.POTGRAD, O.NEAR)
+    assert rel_l2(parts, both) < 1e-14
+
+
+def test_converges_to_direct_sum():
+    pos, q = W.uniform_cube(6000, seed=23)
+    t = O.Tree(pos, 20, 32)
+    tg = np.arange(0, 6000, 7)
+    ref = O.direct(pos, q, tg)
+    errs = []
+    for p in (2, 4, 8, 12):
+        out = t.fmm(p, q, O.POTGRAD)[tg]
+        errs.append((rel_l2(out[:, 0], ref[:, 0]), rel_l2(out[:, 1:], ref[:, 1:])))
+    for k in range(len(errs) - 1):
+        assert errs[k + 1][0] < errs[k][0] and errs[k + 1][1] < errs[k][1]
+    # truncation error of this scheme at p=12 on a uniform cube: ~2.6e-7 (phi), ~1.5e-6 (grad)
+    assert errs[-1][0] < 5e-7 and errs[-1][1] < 3e-6
+
+
+def test_self_generator_matches_default():
+    # FMM3D(partitioner, comm) uses self_generator i -> {i} (fmm.hpp:423-427)
+    pos, q = W.uniform_cube(1500, seed=24)
+    t = O.Tree(pos, 20, 32)
+    row = np.arange(1501, dtype=np.int64)
+    val = np.arange(1500, dtype=np.int64)
+    a = t.fmm(4, q, O.POTGRAD, self_lists=(row, val))
+    b = t.fmm(4, q, O.POTGRAD)
+    assert np.array_equal(a, b)
+
+
+def test_self_generator_far_pairs_are_subtracted():
+    # excluding a far pair (i, j) must remove exactly its direct contribution
+    pos, q = W.uniform_cube(1500, seed=25)
+    t = O.Tree(pos, 20, 32)
+    p = 6
+    i, j = 0, int(np.argmax(np.linalg.norm(pos - pos[0], axis=1)))
+    row = np.zeros(1501, dtype=np.int64)
+    vals = []
+    for k in range(1500):
+        vals.append([k] + ([j] if k == i else []))
+        row[k + 1] = row[k] + len(vals[-1])
+    val = np.array([v for vs in vals for v in vs], dtype=np.int64)
+    a = t.fmm(p, q, O.POT, self_lists=(row, val))
+    b = t.fmm(p, q, O.POT)
+    assert abs((b[i] - a[i]) - q[j] / np.linalg.norm(pos[i] - pos[j])) < 1e-12
+    mask = np.ones(1500, bool)
+    mask[i] = False
+    assert np.array_equal(a[mask], b[mask])
