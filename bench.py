@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Benchmark of the FMM evaluation: potential + gradient of 16M Plummer points.
+
+Workload (default): N = 2^24 points of a Plummer sphere (scale radius 1,
+truncated at r = 50), charges uniform in (0,1], expansion order p = 12, leaf
+capacity 64, adaptive octree; output phi and grad phi in FP64.  One step is one
+evaluation of all N points with charges and outputs resident in HBM.
+
+Prints one JSON line: points/sec, the per-stage device times, the error against
+an FP64 direct sum at 10k sampled targets, the roofline of the dominant kernel
+(M2L) and the reference CPU FMM timed on this host's cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from simplefmm_amd import api, workloads as W  # noqa: E402
+
+PEAK_FP64 = 78.6e12    # MI355X dense FP64 (vector and MFMA), FLOP/s
+PEAK_HBM = 8.0e12      # bytes/s
+
+WORKLOADS = {
+    # name: (generator, n, seed, p, cap)
+    "plummer16m_p12": ("plummer", 1 << 24, 3, 12, 64),
+    "uniform4m_p10": ("uniform_cube", 1 << 22, 1, 10, 64),
+    "uniform100k_p8": ("uniform_cube", 100_000, 1, 8, 64),
+}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def m2l_flops_survey(p: int) -> int:
+    """Per (pair, component) flops of rotate-translate-rotate M2L on complex
+    full-m expansions (8 n_rot + 5 n_coax + 12 (p+1)^2)."""
+    n_rot = (p + 1) * (2 * p + 1) * (2 * p + 3) // 3
+    n_coax = sum((p + 1 - abs(k)) ** 2 for k in range(-p, p + 1))
+    return 8 * n_rot + 5 * n_coax + 12 * (p + 1) ** 2
+
+
+def m2l_flops_real(p: int) -> int:
+    """Per (pair, component) flops of the same M2L on the real form with
+    conjugate symmetry (what the kernel executes): two polar rotations and the
+    coaxial translation, each sum_n (n+1)^2 + n^2 multiply-adds, plus the two
+    azimuthal 2x2 rotations of every (n, m>0)."""
+    mac = sum((n + 1) ** 2 + n * n for n in range(p + 1))
+    azi = 4 * p * (p + 1) // 2
+    return 2 * (3 * mac + 2 * azi)
+
+
+def cpu_baseline(gen: str, p: int, cap: int, sample_n: int, seed: int):
+    """Reference CPU FMM (oracle/_ref, built from the reference sources) on a
+    bounded sample of the workload, all host cores as MPI ranks."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness_v4")
+    mpiexec = "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(exe) and os.path.exists(mpiexec)):
+        return None, "reference CPU build not present"
+    cores = os.cpu_count() or 1
+    pos, q = getattr(W, gen)(sample_n, seed=seed)
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        W.write_harness_input(fin, pos, q)
+        try:
+            r = subprocess.run([mpiexec, "-n", str(cores), exe, fin, "-", str(p), str(cap), "20", "time", "-", "1"],
+                               capture_output=True, text=True, timeout=240)
+        except subprocess.TimeoutExpired:
+            return None, "reference CPU run timed out"
+    if r.returncode != 0:
+        return None, f"reference CPU run failed ({r.returncode}): {r.stderr.strip()[-200:]}"
+    rec = json.loads([ln for ln in r.stdout.splitlines() if ln.startswith("{")][-1])
+    t_eval = rec["rinv_s"] + rec["rinv_curl_s"] + rec["rinv_curl2_s"]
+    return {
+        "value": sample_n / t_eval, "unit": "points/sec", "cores": cores, "kind": "reference",
+        "sample": (f"{gen} N={sample_n} p={p} cap={cap}: reference FMM3D<double,3,{p},Vector,true>, "
+                   f"phi by rinv + grad phi by two rinv<true> (curl identity), {cores} MPI ranks; "
+                   f"eval {t_eval:.2f}s, plan build {rec['ctor_s']:.2f}s, tree {rec['tree_s']:.2f}s"),
+    }, None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="plummer16m_p12", choices=sorted(WORKLOADS))
+    ap.add_argument("--n", type=int, default=0, help="override the particle count")
+    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--targets", type=int, default=10_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 18)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is not built)",
+              file=sys.stderr)
+        return 2
+
+    gen, n, seed, p, cap = WORKLOADS[args.workload]
+    if args.n:
+        n = args.n
+    log(f"[bench] workload {args.workload}: {gen} n={n} p={p} cap={cap}")
+
+    # CPU baseline first, before this process touches the GPU
+    cpu, cpu_note = (None, "skipped (--no-cpu)") if args.no_cpu else cpu_baseline(gen, p, cap, args.cpu_sample, seed)
+    log(f"[bench] cpu baseline: {cpu if cpu else cpu_note}")
+
+    import torch  # device memory and streams only
+
+    if not torch.cuda.is_available():
+        log("[bench] no GPU visible")
+        return 3
+    pos, q = getattr(W, gen)(n, seed=seed)
+    t0 = time.time()
+    part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, cap)
+    t1 = time.time()
+    fmm = api.FMM3D(part, p, ncharge=1, precision=api.FP64)
+    t2 = time.time()
+    info = part.info()
+    log(f"[bench] tree {t1 - t0:.2f}s (depth {info.depth}, {info.n_node} nodes, {info.n_leaf} leaves, "
+        f"{info.m2l_pairs} M2L pairs, {info.near_pairs} near pairs), plan {t2 - t1:.2f}s")
+
+    dev = torch.device("cuda:0")
+    dq = torch.from_numpy(q).to(dev)
+    du = torch.zeros((n, 4), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        fmm.evaluate_device(api.POTGRAD, 1, dq.data_ptr(), du.data_ptr())
+
+    for _ in range(args.warmup):
+        step()
+    fmm.synchronize()
+    torch.cuda.synchronize()
+    fmm.record_stages(args.steps)
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    fmm.synchronize()
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    ms_per_step = (tb - ta) * 1e3 / args.steps
+    stages = fmm.recorded_stage_ms(args.steps)
+    fmm.record_stages(0)
+    mean = {k: float(np.mean([s[k] for s in stages])) for k in api.STAGES}
+    log(f"[bench] {ms_per_step:.2f} ms/step; stages " + ", ".join(f"{k} {v:.2f}" for k, v in mean.items()))
+
+    # accuracy: one more evaluation into a zeroed buffer vs the FP64 direct sum
+    du.zero_()
+    torch.cuda.synchronize()
+    step()
+    fmm.synchronize()
+    U = du.cpu().numpy()
+    tg = np.sort(np.random.default_rng(12345).choice(n, min(args.targets, n), replace=False))
+    ref = fmm.direct(api.POTGRAD, q, tg)
+
+    def rel(a, b):
+        return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+    err_phi = rel(U[tg, 0], ref[:, 0])
+    err_grad = rel(U[tg, 1:], ref[:, 1:])
+    log(f"[bench] rel L2 vs direct on {len(tg)} targets: phi {err_phi:.3e} grad {err_grad:.3e}")
+
+    m2l_ms = mean["M2L"]
+    flops_survey = info.m2l_pairs * 4 * m2l_flops_survey(p)
+    flops_real = info.m2l_pairs * 4 * m2l_flops_real(p)
+    achieved = flops_survey / (m2l_ms * 1e-3)
+    pts = n / (ms_per_step * 1e-3)
+    line = {
+        "metric": "points/sec (pot+grad) at rel err <=1e-6: mixed vs FP64 GPU, 1 MI355X",
+        "value": pts,
+        "unit": "points/sec",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "precision_path": args.precision,
+        "config": {"workload": args.workload, "generator": gen, "n": n, "p": p, "leaf_capacity": cap,
+                   "max_level": 20, "output": "phi+grad", "parallelism": "single GPU"},
+        "error": {"targets": int(len(tg)), "rel_l2_phi": err_phi, "rel_l2_grad": err_grad,
+                  "within_1e-6": bool(max(err_phi, err_grad) <= 1e-6)},
+        "stages_ms": mean,
+        "tree": {"depth": info.depth, "nodes": info.n_node, "leaves": info.n_leaf, "m2l_pairs": info.m2l_pairs,
+                 "near_pairs": info.near_pairs, "tree_build_s": t1 - t0, "plan_build_s": t2 - t1},
+        "roofline": {"kernel": "m2l", "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
+                     "flops_per_launch": flops_survey, "avg_launch_ms": m2l_ms,
+                     "executed_flops_per_launch": flops_real,
+                     "executed_frac": flops_real / (m2l_ms * 1e-3) / PEAK_FP64},
+        "cpu_baseline": cpu if cpu else {"value": None, "note": cpu_note},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

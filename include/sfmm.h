@@ -113,6 +113,13 @@ int sfmm_synchronize(sfmm_plan plan);
  * [0] gather, [1] P2M, [2] M2M, [3] M2L, [4] L2L, [5] L2P, [6] P2P, [7] scatter, [8] total. */
 int sfmm_last_stage_ms(sfmm_plan plan, float ms[9]);
 
+/* Records the stage times of each of the next `count` evaluations in its own set of
+ * HIP events on the plan's stream (instead of only the last one); count <= 0 stops. */
+int sfmm_record_stages(sfmm_plan plan, int count);
+/* Stage times (as sfmm_last_stage_ms, 9 per evaluation) of the evaluations recorded
+ * since sfmm_record_stages; ms holds room for `capacity` evaluations. */
+int sfmm_recorded_stage_ms(sfmm_plan plan, float* ms, int capacity, int* recorded);
+
 /* FP64 direct sum over all sources j != i for the given original-order targets
  * (main.cpp:70-96): out[nout*t + c] = value (overwritten).  Host arrays, synchronous. */
 int sfmm_direct(sfmm_plan plan, int kind, int ncharge, const double* Q, int64_t ntarget, const int64_t* targets,

@@ -71,6 +71,9 @@ SIGNATURES = {
     "sfmm_plan_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "sfmm_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "sfmm_last_stage_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "sfmm_record_stages": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "sfmm_recorded_stage_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int)]),
     "sfmm_direct": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int64, _ip, _dp]),
     "sfmm_last_error": (ctypes.c_char_p, []),
 }
@@ -265,6 +268,17 @@ class FMM3D:
         ms = (ctypes.c_float * 9)()
         _check(lib().sfmm_last_stage_ms(self.h, ms))
         return dict(zip(STAGES, [float(x) for x in ms]))
+
+    def record_stages(self, count: int):
+        """Keeps per-stage HIP-event times for each of the next `count` evaluations."""
+        _check(lib().sfmm_record_stages(self.h, int(count)))
+
+    def recorded_stage_ms(self, capacity: int) -> list:
+        ms = (ctypes.c_float * (9 * max(capacity, 1)))()
+        rec = ctypes.c_int()
+        _check(lib().sfmm_recorded_stage_ms(self.h, ms, int(capacity), ctypes.byref(rec)))
+        k = min(rec.value, capacity)
+        return [dict(zip(STAGES, [float(ms[9 * r + s]) for s in range(9)])) for r in range(k)]
 
     def direct(self, kind: int, Q, targets, ncharge: int = 1):
         """FP64 direct sum at original-index targets (all sources j != i)."""

@@ -63,6 +63,9 @@ struct sfmm_plan_s {
     int64_t stage_q = 0, stage_u = 0;
     hipEvent_t ev[9] = {};
     bool timed = false;
+    hipEvent_t* last_ev = nullptr;   // event set of the last evaluation
+    std::vector<hipEvent_t> rec_ev;  // sets of 9 events, one set per recorded evaluation
+    int rec_count = 0, rec_used = 0;
 
     ~sfmm_plan_s()
     {
@@ -75,6 +78,7 @@ struct sfmm_plan_s {
         (void)hipFree(hU);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : rec_ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -161,30 +165,33 @@ int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int
     const sfmm::DevPlan& d = P->d;
     hipStream_t s = P->stream;
     const int depth = P->tree->t.depth;
-    HIP_TRY(hipEventRecord(P->ev[0], s));
+    hipEvent_t* ev = P->ev;
+    if (P->rec_used < P->rec_count) ev = &P->rec_ev[9 * static_cast<size_t>(P->rec_used++)];
+    HIP_TRY(hipEventRecord(ev[0], s));
     sfmm::launch_gather_charges(d, qdim, dQ, P->qbuf, s);
     sfmm::launch_fill(P->obuf, static_cast<int64_t>(nc) * d.n, 0.0, s);
-    HIP_TRY(hipEventRecord(P->ev[1], s));
+    HIP_TRY(hipEventRecord(ev[1], s));
     const bool far = (which & SFMM_FAR) && depth >= 2;  // fmm.hpp:816-820
     if (far) sfmm::launch_p2m(d, mode, nc, P->qbuf, P->M, s);
-    HIP_TRY(hipEventRecord(P->ev[2], s));
+    HIP_TRY(hipEventRecord(ev[2], s));
     if (far)
         for (int l = depth - 1; l >= 2; --l)
             sfmm::launch_m2m(d, {P->level_first[l], P->level_first[l + 1] - P->level_first[l]}, nc, P->M, s);
-    HIP_TRY(hipEventRecord(P->ev[3], s));
+    HIP_TRY(hipEventRecord(ev[3], s));
     if (far) sfmm::launch_m2l(d, nc, P->M, P->L, s);
-    HIP_TRY(hipEventRecord(P->ev[4], s));
+    HIP_TRY(hipEventRecord(ev[4], s));
     if (far)
         for (int l = 3; l <= depth; ++l)
             sfmm::launch_l2l(d, {P->level_first[l], P->level_first[l + 1] - P->level_first[l]}, nc, P->L, s);
-    HIP_TRY(hipEventRecord(P->ev[5], s));
+    HIP_TRY(hipEventRecord(ev[5], s));
     if (far) sfmm::launch_l2p(d, nc, P->L, P->obuf, s);
-    HIP_TRY(hipEventRecord(P->ev[6], s));
+    HIP_TRY(hipEventRecord(ev[6], s));
     if (which & SFMM_NEAR) sfmm::launch_p2p(d, mode, nc, P->qbuf, P->obuf, s);
-    HIP_TRY(hipEventRecord(P->ev[7], s));
+    HIP_TRY(hipEventRecord(ev[7], s));
     sfmm::launch_scatter_outputs(d, nc, P->obuf, dU, s);
-    HIP_TRY(hipEventRecord(P->ev[8], s));
+    HIP_TRY(hipEventRecord(ev[8], s));
     HIP_TRY(hipGetLastError());
+    P->last_ev = ev;
     P->timed = true;
     return SFMM_OK;
 }
@@ -489,9 +496,43 @@ int sfmm_last_stage_ms(sfmm_plan plan, float ms[9])
     if (!plan || !ms) return fail(SFMM_ERR_ARG, "null argument");
     if (!plan->timed) return fail(SFMM_ERR_STATE, "no evaluation has run yet");
     HIP_TRY(hipSetDevice(plan->device));
-    HIP_TRY(hipEventSynchronize(plan->ev[8]));
-    for (int k = 0; k < 8; ++k) HIP_TRY(hipEventElapsedTime(&ms[k], plan->ev[k], plan->ev[k + 1]));
-    HIP_TRY(hipEventElapsedTime(&ms[8], plan->ev[0], plan->ev[8]));
+    const hipEvent_t* ev = plan->last_ev;
+    HIP_TRY(hipEventSynchronize(ev[8]));
+    for (int k = 0; k < 8; ++k) HIP_TRY(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+    HIP_TRY(hipEventElapsedTime(&ms[8], ev[0], ev[8]));
+    return SFMM_OK;
+}
+
+int sfmm_record_stages(sfmm_plan plan, int count)
+{
+    if (!plan) return fail(SFMM_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(plan->device));
+    HIP_TRY(hipStreamSynchronize(plan->stream));
+    const size_t need = 9 * static_cast<size_t>(std::max(count, 0));
+    if (need > plan->rec_ev.size()) plan->last_ev = plan->ev;  // the vector may reallocate
+    plan->rec_ev.reserve(need);
+    while (plan->rec_ev.size() < need) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        plan->rec_ev.push_back(e);
+    }
+    plan->rec_count = std::max(count, 0);
+    plan->rec_used = 0;
+    return SFMM_OK;
+}
+
+int sfmm_recorded_stage_ms(sfmm_plan plan, float* ms, int capacity, int* recorded)
+{
+    if (!plan || !recorded || (capacity > 0 && !ms)) return fail(SFMM_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(plan->device));
+    const int nrec = std::min(plan->rec_used, capacity);
+    for (int r = 0; r < nrec; ++r) {
+        hipEvent_t* ev = &plan->rec_ev[9 * static_cast<size_t>(r)];
+        HIP_TRY(hipEventSynchronize(ev[8]));
+        for (int k = 0; k < 8; ++k) HIP_TRY(hipEventElapsedTime(&ms[9 * r + k], ev[k], ev[k + 1]));
+        HIP_TRY(hipEventElapsedTime(&ms[9 * r + 8], ev[0], ev[8]));
+    }
+    *recorded = plan->rec_used;
     return SFMM_OK;
 }
 
