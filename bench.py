@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 from simplefmm_amd import api, workloads as W  # noqa: E402
 
 PEAK_FP64 = 78.6e12    # MI355X dense FP64 (vector and MFMA), FLOP/s
+PEAK_FP32 = 157.3e12   # MI355X dense FP32 MFMA, FLOP/s
 PEAK_HBM = 8.0e12      # bytes/s
 
 WORKLOADS = {
@@ -96,7 +97,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plummer16m_p12", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=0, help="override the particle count")
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--targets", type=int, default=10_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
@@ -124,85 +125,101 @@ def main() -> int:
     t0 = time.time()
     part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, cap)
     t1 = time.time()
-    fmm = api.FMM3D(part, p, ncharge=1, precision=api.FP64)
-    t2 = time.time()
     info = part.info()
     log(f"[bench] tree {t1 - t0:.2f}s (depth {info.depth}, {info.n_node} nodes, {info.n_leaf} leaves, "
-        f"{info.m2l_pairs} M2L pairs, {info.near_pairs} near pairs), plan {t2 - t1:.2f}s")
+        f"{info.m2l_pairs} M2L pairs, {info.near_pairs} near pairs)")
 
     dev = torch.device("cuda:0")
     dq = torch.from_numpy(q).to(dev)
     du = torch.zeros((n, 4), dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-
-    def step():
-        fmm.evaluate_device(api.POTGRAD, 1, dq.data_ptr(), du.data_ptr())
-
-    for _ in range(args.warmup):
-        step()
-    fmm.synchronize()
-    torch.cuda.synchronize()
-    fmm.record_stages(args.steps)
-    torch.cuda.synchronize()
-    ta = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    fmm.synchronize()
-    torch.cuda.synchronize()
-    tb = time.perf_counter()
-    ms_per_step = (tb - ta) * 1e3 / args.steps
-    stages = fmm.recorded_stage_ms(args.steps)
-    fmm.record_stages(0)
-    mean = {k: float(np.mean([s[k] for s in stages])) for k in api.STAGES}
-    log(f"[bench] {ms_per_step:.2f} ms/step; stages " + ", ".join(f"{k} {v:.2f}" for k, v in mean.items()))
-
-    # accuracy: one more evaluation into a zeroed buffer vs the FP64 direct sum
-    du.zero_()
-    torch.cuda.synchronize()
-    step()
-    fmm.synchronize()
-    U = du.cpu().numpy()
     tg = np.sort(np.random.default_rng(12345).choice(n, min(args.targets, n), replace=False))
-    ref = fmm.direct(api.POTGRAD, q, tg)
+    ref = None
 
     def rel(a, b):
         return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
 
-    err_phi = rel(U[tg, 0], ref[:, 0])
-    err_grad = rel(U[tg, 1:], ref[:, 1:])
-    log(f"[bench] rel L2 vs direct on {len(tg)} targets: phi {err_phi:.3e} grad {err_grad:.3e}")
+    def run_path(prec: str) -> dict:
+        nonlocal ref
+        t_a = time.time()
+        fmm = api.FMM3D(part, p, ncharge=1, precision=api.MIXED if prec == "mixed" else api.FP64)
+        t_b = time.time()
 
-    m2l_ms = mean["M2L"]
-    flops_survey = info.m2l_pairs * 4 * m2l_flops_survey(p)
-    flops_real = info.m2l_pairs * 4 * m2l_flops_real(p)
-    achieved = flops_survey / (m2l_ms * 1e-3)
-    pts = n / (ms_per_step * 1e-3)
+        def step():
+            fmm.evaluate_device(api.POTGRAD, 1, dq.data_ptr(), du.data_ptr())
+
+        for _ in range(args.warmup):
+            step()
+        fmm.synchronize()
+        torch.cuda.synchronize()
+        fmm.record_stages(args.steps)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        fmm.synchronize()
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        ms = (tb - ta) * 1e3 / args.steps
+        stages = fmm.recorded_stage_ms(args.steps)
+        fmm.record_stages(0)
+        mean = {k: float(np.mean([s_[k] for s_ in stages])) for k in api.STAGES}
+        log(f"[bench] {prec}: {ms:.2f} ms/step; stages " + ", ".join(f"{k} {v:.2f}" for k, v in mean.items()))
+        # accuracy: one more evaluation into a zeroed buffer vs the FP64 direct sum
+        du.zero_()
+        torch.cuda.synchronize()
+        step()
+        fmm.synchronize()
+        U = du.cpu().numpy()
+        if ref is None:
+            ref = fmm.direct(api.POTGRAD, q, tg)
+        err_phi = rel(U[tg, 0], ref[:, 0])
+        err_grad = rel(U[tg, 1:], ref[:, 1:])
+        log(f"[bench] {prec}: rel L2 vs direct on {len(tg)} targets: phi {err_phi:.3e} grad {err_grad:.3e}")
+        m2l_ms = mean["M2L"]
+        peak = PEAK_FP32 if prec == "mixed" else PEAK_FP64
+        flops_survey = info.m2l_pairs * 4 * m2l_flops_survey(p)
+        flops_real = info.m2l_pairs * 4 * m2l_flops_real(p)
+        achieved = flops_survey / (m2l_ms * 1e-3)
+        del fmm
+        return {
+            "points_per_sec": n / (ms * 1e-3), "ms_per_step": ms, "plan_build_s": t_b - t_a,
+            "error": {"targets": int(len(tg)), "rel_l2_phi": err_phi, "rel_l2_grad": err_grad,
+                      "within_1e-6": bool(max(err_phi, err_grad) <= 1e-6)},
+            "stages_ms": mean,
+            "roofline": {"kernel": "m2l", "bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                         "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                         "flops_per_launch": flops_survey, "avg_launch_ms": m2l_ms,
+                         "executed_flops_per_launch": flops_real,
+                         "executed_frac": flops_real / (m2l_ms * 1e-3) / peak},
+        }
+
+    paths = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    res = {pr: run_path(pr) for pr in paths}
+    head = res["mixed"] if "mixed" in res else res["fp64"]
     line = {
         "metric": "points/sec (pot+grad) at rel err <=1e-6: mixed vs FP64 GPU, 1 MI355X",
-        "value": pts,
+        "value": head["points_per_sec"],
         "unit": "points/sec",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if "mixed" not in res else "f64+f32",
         "data": "synthetic",
-        "precision_path": args.precision,
+        "precision_path": "mixed" if "mixed" in res else "fp64",
         "config": {"workload": args.workload, "generator": gen, "n": n, "p": p, "leaf_capacity": cap,
                    "max_level": 20, "output": "phi+grad", "parallelism": "single GPU"},
-        "error": {"targets": int(len(tg)), "rel_l2_phi": err_phi, "rel_l2_grad": err_grad,
-                  "within_1e-6": bool(max(err_phi, err_grad) <= 1e-6)},
-        "stages_ms": mean,
+        "error": head["error"],
+        "stages_ms": head["stages_ms"],
+        "roofline": head["roofline"],
+        "paths": res,
+        "speedup_mixed_vs_fp64": (res["mixed"]["points_per_sec"] / res["fp64"]["points_per_sec"]
+                                  if len(res) == 2 else None),
         "tree": {"depth": info.depth, "nodes": info.n_node, "leaves": info.n_leaf, "m2l_pairs": info.m2l_pairs,
-                 "near_pairs": info.near_pairs, "tree_build_s": t1 - t0, "plan_build_s": t2 - t1},
-        "roofline": {"kernel": "m2l", "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
-                     "flops_per_launch": flops_survey, "avg_launch_ms": m2l_ms,
-                     "executed_flops_per_launch": flops_real,
-                     "executed_frac": flops_real / (m2l_ms * 1e-3) / PEAK_FP64},
+                 "near_pairs": info.near_pairs, "tree_build_s": t1 - t0},
         "cpu_baseline": cpu if cpu else {"value": None, "note": cpu_note},
     }
     print(json.dumps(line), flush=True)

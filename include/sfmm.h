@@ -83,13 +83,14 @@ int sfmm_tree_level_lists(sfmm_tree tree, int level, int64_t* c_node, int64_t* l
 
 typedef struct {
     int p;                    /* expansion order, 1..16 (FMM3D template p)            */
-    int precision;            /* SFMM_FP64 or SFMM_MIXED                              */
+    int precision;            /* SFMM_FP64 or SFMM_MIXED (FP32 MFMA M2L stages, FP64 accumulation) */
     int max_ncharge;          /* largest ncharge later passed to SFMM_POTENTIAL (1..4) */
     /* optional self exclusions (FMM3D(partitioner, comm, self_generator), fmm.hpp:204):
      * for particle i, pairs (i, j), j in self_val[self_row[i] .. self_row[i+1]), are
      * excluded; NULL means j == i only. */
     const int64_t* self_row;
     const int64_t* self_val;
+    int m2l_kernel;           /* 0: offset-grouped MFMA (default); 1: row-parallel VALU kernel */
 } sfmm_plan_desc;
 
 /* FMM3D(partitioner, comm[, self_generator]): fmm.hpp:204-427.  Uploads the tree

@@ -53,7 +53,7 @@ class TreeInfo(ctypes.Structure):
 
 class PlanDesc(ctypes.Structure):
     _fields_ = [("p", ctypes.c_int), ("precision", ctypes.c_int), ("max_ncharge", ctypes.c_int),
-                ("self_row", _ip), ("self_val", _ip)]
+                ("self_row", _ip), ("self_val", _ip), ("m2l_kernel", ctypes.c_int)]
 
 
 # symbol -> (restype, argtypes); every function declared in include/sfmm.h
@@ -205,7 +205,8 @@ class FMM3D:
     rinv_potgrad(q, U) accumulates (phi, grad phi) of scalar charges, U (n, 4).
     """
 
-    def __init__(self, partitioner: OctreePartitioner, p: int, ncharge: int = 3, precision: int = FP64):
+    def __init__(self, partitioner: OctreePartitioner, p: int, ncharge: int = 3, precision: int = FP64,
+                 m2l_kernel: int = 0):
         if partitioner.h is None:
             raise SfmmError("refine() the partitioner before building an FMM3D")
         self.partitioner = partitioner  # the plan references the tree: keep it alive
@@ -215,6 +216,7 @@ class FMM3D:
         desc.p = self.p
         desc.precision = int(precision)
         desc.max_ncharge = max(1, min(4, self.ncharge))
+        desc.m2l_kernel = int(m2l_kernel)
         h = ctypes.c_void_p()
         _check(lib().sfmm_plan_create(partitioner.h, ctypes.byref(desc), ctypes.byref(h)))
         self.h = h

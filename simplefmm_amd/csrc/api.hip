@@ -63,6 +63,12 @@ struct sfmm_plan_s {
     int64_t stage_q = 0, stage_u = 0;
     hipEvent_t ev[9] = {};
     bool timed = false;
+    int m2l_kernel = 0;
+    sfmm::M2LArgs m2l[5];              // offset-grouped M2L work per component count (built on first use)
+    const double* rotA64 = nullptr;
+    const double* coaxA64 = nullptr;
+    const float* rotA32 = nullptr;
+    const float* coaxA32 = nullptr;
     hipEvent_t* last_ev = nullptr;   // event set of the last evaluation
     std::vector<hipEvent_t> rec_ev;  // sets of 9 events, one set per recorded evaluation
     int rec_count = 0, rec_used = 0;
@@ -152,6 +158,64 @@ int ensure_buffers(sfmm_plan_s* P, int nc, int qdim)
     return SFMM_OK;
 }
 
+// Groups the targets of each level by child octant into blocks of 16/nc and
+// lists, per block, the offsets that occur with the source of every target.
+int build_m2l_blocks(sfmm_plan_s* P, int nc)
+{
+    sfmm::M2LArgs& A = P->m2l[nc];
+    if (A.blk_row) return SFMM_OK;
+    const sfmm::Tree& t = P->tree->t;
+    const int TB = 16 / nc;
+    std::vector<int32_t> blk_tgt, ent_src, src(343 * TB);
+    std::vector<int64_t> blk_row{0};
+    std::vector<int16_t> ent_off;
+    std::vector<char> used(343, 0);
+    std::vector<int32_t> nodes;
+    for (int l = 2; l <= t.depth; ++l) {
+        const auto& lv = t.lv[l];
+        for (int o = 0; o < 8; ++o) {
+            nodes.clear();
+            for (int32_t a = 0; a < lv.n_node(); ++a)
+                if ((lv.code[a] & 7) == o) nodes.push_back(a);
+            for (size_t b = 0; b < nodes.size(); b += TB) {
+                std::fill(src.begin(), src.end(), -1);
+                for (int tt = 0; tt < TB; ++tt) {
+                    if (b + tt >= nodes.size()) {
+                        blk_tgt.push_back(-1);
+                        continue;
+                    }
+                    const int32_t a = nodes[b + tt];
+                    blk_tgt.push_back(static_cast<int32_t>(P->level_first[l] + a));
+                    for (int64_t k = lv.il_row[a]; k < lv.il_row[a + 1]; ++k) {
+                        const int off = lv.il_off[k];
+                        src[off * TB + tt] = static_cast<int32_t>(P->level_first[l] + lv.il_src[k]);
+                        used[off] = 1;
+                    }
+                }
+                for (int off = 0; off < 343; ++off) {
+                    if (!used[off]) continue;
+                    used[off] = 0;
+                    ent_off.push_back(static_cast<int16_t>(off));
+                    ent_src.insert(ent_src.end(), src.begin() + off * TB, src.begin() + (off + 1) * TB);
+                }
+                blk_row.push_back(static_cast<int64_t>(ent_off.size()));
+            }
+        }
+    }
+    A.n_block = static_cast<int64_t>(blk_row.size()) - 1;
+    int rc;
+    if ((rc = P->upload(blk_tgt, &A.blk_tgt)) || (rc = P->upload(blk_row, &A.blk_row)) ||
+        (rc = P->upload(ent_off, &A.ent_off)) || (rc = P->upload(ent_src, &A.ent_src)))
+        return rc;
+    A.fp32 = P->precision == SFMM_MIXED;
+    A.rotA = A.fp32 ? static_cast<const void*>(P->rotA32) : static_cast<const void*>(P->rotA64);
+    A.coaxA = A.fp32 ? static_cast<const void*>(P->coaxA32) : static_cast<const void*>(P->coaxA64);
+    A.off_class = P->d.off_class;
+    A.off_d2 = P->d.off_d2;
+    A.off_cs = P->d.off_cs;
+    return SFMM_OK;
+}
+
 int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int which)
 {
     sfmm::Mode mode;
@@ -162,6 +226,8 @@ int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int
         return fail(SFMM_ERR_ARG, "ncharge exceeds the plan's max_ncharge");
     HIP_TRY(hipSetDevice(P->device));
     if (int rc = ensure_buffers(P, nc, qdim)) return rc;
+    if (P->m2l_kernel == 0)
+        if (int rc = build_m2l_blocks(P, nc)) return rc;
     const sfmm::DevPlan& d = P->d;
     hipStream_t s = P->stream;
     const int depth = P->tree->t.depth;
@@ -178,7 +244,10 @@ int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int
         for (int l = depth - 1; l >= 2; --l)
             sfmm::launch_m2m(d, {P->level_first[l], P->level_first[l + 1] - P->level_first[l]}, nc, P->M, s);
     HIP_TRY(hipEventRecord(ev[3], s));
-    if (far) sfmm::launch_m2l(d, nc, P->M, P->L, s);
+    if (far) {
+        if (P->m2l_kernel == 0) sfmm::launch_m2l_mfma(P->m2l[nc], P->p, nc, P->M, P->L, s);
+        else sfmm::launch_m2l(d, nc, P->M, P->L, s);
+    }
     HIP_TRY(hipEventRecord(ev[4], s));
     if (far)
         for (int l = 3; l <= depth; ++l)
@@ -287,7 +356,10 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
     if (!tree || !desc || !out) return fail(SFMM_ERR_ARG, "null argument");
     *out = nullptr;
     if (desc->p < 1 || desc->p > 16) return fail(SFMM_ERR_ARG, "expansion order p must be in 1..16");
-    if (desc->precision != SFMM_FP64) return fail(SFMM_ERR_ARG, "only SFMM_FP64 is available in this build");
+    if (desc->precision != SFMM_FP64 && desc->precision != SFMM_MIXED) return fail(SFMM_ERR_ARG, "unknown precision");
+    if (desc->m2l_kernel != 0 && desc->m2l_kernel != 1) return fail(SFMM_ERR_ARG, "m2l_kernel must be 0 or 1");
+    if (desc->precision == SFMM_MIXED && desc->m2l_kernel != 0)
+        return fail(SFMM_ERR_ARG, "mixed precision runs on the MFMA M2L kernel");
     if (desc->self_row || desc->self_val) return fail(SFMM_ERR_ARG, "custom self exclusions are not available in this build");
     if (desc->max_ncharge < 1 || desc->max_ncharge > 4) return fail(SFMM_ERR_ARG, "max_ncharge must be in 1..4");
     auto P = std::make_unique<sfmm_plan_s>();
@@ -296,6 +368,7 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
     P->K = (desc->p + 1) * (desc->p + 1);
     P->precision = desc->precision;
     P->max_ncharge = desc->max_ncharge;
+    P->m2l_kernel = desc->m2l_kernel;
     HIP_TRY(hipGetDevice(&P->device));
     HIP_TRY(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
     for (auto& e : P->ev) HIP_TRY(hipEventCreate(&e));
@@ -426,6 +499,16 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
     UP(tb.off_class, off_class); UP(tb.off_d2, off_d2); UP(tb.om, om); UP(om_rows, om_rows);
     UP(tb.rot_off, rot_off); UP(tb.coax_off, coax_off); UP(rec, rec);
 #undef UP
+    {
+        std::vector<double> rotA, coaxA;
+        sfmm::build_m2l_fragments(tb, rotA, coaxA);
+        if (P->precision == SFMM_MIXED) {
+            std::vector<float> r32(rotA.begin(), rotA.end()), c32(coaxA.begin(), coaxA.end());
+            if ((rc = P->upload(r32, &P->rotA32)) || (rc = P->upload(c32, &P->coaxA32))) return rc;
+        } else {
+            if ((rc = P->upload(rotA, &P->rotA64)) || (rc = P->upload(coaxA, &P->coaxA64))) return rc;
+        }
+    }
     *out = P.release();
     return SFMM_OK;
 }

@@ -53,6 +53,22 @@ struct DevPlan {
     const double* rec = nullptr;  // recurrence coefficients, see kernels.hip
 };
 
+// Offset-grouped M2L work (see m2l_mfma.hip): blocks of 16/nc targets of one
+// level and child octant, each with its list of (offset, 16/nc sources).
+struct M2LArgs {
+    int64_t n_block = 0;
+    const int32_t* blk_tgt = nullptr;  // [n_block][16/nc] target gnodes, -1 = none
+    const int64_t* blk_row = nullptr;  // [n_block+1] entry ranges
+    const int16_t* ent_off = nullptr;  // offset id per entry
+    const int32_t* ent_src = nullptr;  // [entries][16/nc] source gnodes, -1 = none
+    const void* rotA = nullptr;        // rotation fragments [70][2][nfrag][64], double or float
+    const void* coaxA = nullptr;       // coaxial fragments [28][nfrag][64]
+    const int32_t* off_class = nullptr;
+    const int32_t* off_d2 = nullptr;
+    const double* off_cs = nullptr;
+    int fp32 = 0;                      // 1: FP32 MFMA stages (mixed precision)
+};
+
 struct LevelSpan {
     int64_t first = 0, count = 0;  // gnode range of one level
 };
@@ -65,6 +81,9 @@ void launch_m2m(const DevPlan& d, LevelSpan parents, int nc, double* M, hipStrea
 void launch_m2l(const DevPlan& d, int nc, const double* M, double* L, hipStream_t s);
 void launch_l2l(const DevPlan& d, LevelSpan children, int nc, double* L, hipStream_t s);
 void launch_l2p(const DevPlan& d, int nc, const double* L, double* out, hipStream_t s);
+void launch_m2l_mfma(const M2LArgs& a, int p, int nc, const double* M, double* L, hipStream_t s);
+int m2l_rot_nfrag(int p);
+int m2l_coax_nfrag(int p);
 void launch_p2p(const DevPlan& d, Mode mode, int nc, const double* q, double* out, hipStream_t s);
 void launch_direct(const DevPlan& d, Mode mode, int nc, const double* q, int64_t ntarget, const int64_t* targets,
                    double* out, double* partial, int nsplit, hipStream_t s);

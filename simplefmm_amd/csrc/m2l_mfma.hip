@@ -1,0 +1,19 @@
+// Dispatch of the offset-grouped MFMA M2L (kernels in m2l_mfma.hpp).
+#include "m2l_mfma.hpp"
+
+namespace sfmm {
+
+int m2l_rot_nfrag(int p) { return rot_nfrag(p); }
+int m2l_coax_nfrag(int p) { return coax_nfrag(p); }
+
+void launch_m2l_mfma(const M2LArgs& a, int p, int nc, const double* M, double* L, hipStream_t s)
+{
+    if (a.n_block == 0) return;
+    if (a.fp32) {
+        if (!m2l_f32_lo(a, p, nc, M, L, s) && !m2l_f32_hi(a, p, nc, M, L, s)) m2l_f32_top(a, p, nc, M, L, s);
+    } else {
+        if (!m2l_f64_lo(a, p, nc, M, L, s) && !m2l_f64_hi(a, p, nc, M, L, s)) m2l_f64_top(a, p, nc, M, L, s);
+    }
+}
+
+}  // namespace sfmm

@@ -1,0 +1,105 @@
+// Tiling of the offset-grouped M2L onto 16x16x4 MFMA tiles, shared by the host
+// (which lays out the A-operand fragments) and the kernel (which consumes them).
+//
+// For one interaction offset the rotate-translate-rotate operator is three
+// block-diagonal products applied to 16 stacked columns (targets x components):
+//   stage 1  Y = P X   per degree n: cos block rows a=0..n, cols b=0..n;
+//                      sin block rows a=0..n (row 0 zero), cols b=1..n
+//   stage 2  Z = C Y   per order k: rows j=k..p, cols n=k..p (cos and sin columns
+//                      share the block)
+//   stage 3  L = Q Z   per degree j, same block shapes as stage 1
+// A block is cut into 16-row m-tiles and 4-column k-steps; one "fragment" is the
+// 16x4 A operand of one MFMA, stored as 64 values in lane order
+// (lane l holds A[l % 16][l / 16]).
+//
+// Column-stacked buffers live in LDS in two row orders:
+//   ps(n, b, part): degree-major, per degree the n+1 cos rows then the n sin rows
+//   om(k, n, part): order-major, per order k the cos rows n=k..p then the sin rows
+#pragma once
+
+#if !defined(__HIPCC__) && !defined(__host__)
+#define __host__
+#define __device__
+#endif
+
+namespace sfmm {
+
+__host__ __device__ constexpr int ceil_div(int a, int b) { return (a + b - 1) / b; }
+__host__ __device__ constexpr int ps_row(int n, int b, int part) { return part == 0 ? n * n + b : n * n + n + b; }
+__host__ __device__ constexpr int om_base(int P, int k) { return k == 0 ? 0 : (P + 1) + 2 * ((k - 1) * (P + 1) - (k - 1) * k / 2); }
+__host__ __device__ constexpr int om_row(int P, int k, int n, int part)
+{
+    return om_base(P, k) + (part == 0 ? 0 : (P + 1 - k)) + (n - k);
+}
+
+// rotation blocks (stages 1 and 3): k-steps of one (degree, part) tile
+__host__ __device__ constexpr int rot_nks(int n, int part) { return part == 0 ? ceil_div(n + 1, 4) : ceil_div(n, 4); }
+__host__ __device__ constexpr int rot_mts(int n) { return ceil_div(n + 1, 16); }
+// fragment index of (n, part, mt, ks=0) inside one direction of one polar class
+__host__ __device__ constexpr int rot_frag(int n, int part, int mt)
+{
+    int f = 0;
+    for (int q = 0; q < n; ++q) f += rot_mts(q) * (rot_nks(q, 0) + rot_nks(q, 1));
+    if (part == 1) f += rot_mts(n) * rot_nks(n, 0);
+    return f + mt * rot_nks(n, part);
+}
+__host__ __device__ constexpr int rot_nfrag(int P) { return rot_frag(P + 1, 0, 0); }
+
+// coaxial blocks (stage 2)
+__host__ __device__ constexpr int coax_nks(int P, int k) { return ceil_div(P + 1 - k, 4); }
+__host__ __device__ constexpr int coax_mts(int P, int k) { return ceil_div(P + 1 - k, 16); }
+__host__ __device__ constexpr int coax_frag(int P, int k, int mt)
+{
+    int f = 0;
+    for (int q = 0; q < k; ++q) f += coax_mts(P, q) * coax_nks(P, q);
+    return f + mt * coax_nks(P, k);
+}
+__host__ __device__ constexpr int coax_nfrag(int P) { return coax_frag(P, P + 1, 0); }
+
+// Static work split over the 4 wavefronts of a workgroup: longest tiles first,
+// each to the least loaded wave.
+struct WaveTiles {
+    int count[4] = {0, 0, 0, 0};
+    int item[4][48] = {};  // packed (index << 8) | (part << 4) | mt
+};
+
+template <int P>
+struct M2LSchedule {
+    WaveTiles s1, s2, s3;  // s3 items are degrees j (both parts, all m-tiles)
+
+    constexpr M2LSchedule()
+    {
+        int load[4] = {0, 0, 0, 0};
+        auto pick = [&](int cost) {
+            int w = 0;
+            for (int v = 1; v < 4; ++v)
+                if (load[v] < load[w]) w = v;
+            load[w] += cost;
+            return w;
+        };
+        for (int n = P; n >= 0; --n)
+            for (int part = 0; part < 2; ++part) {
+                if (part == 1 && n == 0) continue;
+                for (int mt = 0; mt < rot_mts(n); ++mt) {
+                    const int w = pick(rot_nks(n, part));
+                    s1.item[w][s1.count[w]++] = (n << 8) | (part << 4) | mt;
+                }
+            }
+        for (int v = 0; v < 4; ++v) load[v] = 0;
+        for (int k = 0; k <= P; ++k)
+            for (int part = 0; part < 2; ++part) {
+                if (part == 1 && k == 0) continue;
+                for (int mt = 0; mt < coax_mts(P, k); ++mt) {
+                    const int w = pick(coax_nks(P, k));
+                    s2.item[w][s2.count[w]++] = (k << 8) | (part << 4) | mt;
+                }
+            }
+        for (int v = 0; v < 4; ++v) load[v] = 0;
+        for (int j = P; j >= 0; --j) {
+            const int w = pick(rot_mts(j) * (rot_nks(j, 0) + rot_nks(j, 1)));
+            s3.item[w][s3.count[w]++] = j << 8;
+        }
+    }
+};
+
+}  // namespace sfmm

@@ -6,6 +6,8 @@
 // restated here in the level-scaled real form.
 #include "tables.hpp"
 
+#include "m2l_sched.hpp"
+
 #include <cmath>
 #include <complex>
 
@@ -106,6 +108,51 @@ void complex_to_real_form(int p, const std::vector<double>& Tre, const std::vect
                 }
             }
         }
+}
+
+void build_m2l_fragments(const Tables& t, std::vector<double>& rotA, std::vector<double>& coaxA)
+{
+    const int P = t.p;
+    const int nrf = rot_nfrag(P), ncf = coax_nfrag(P);
+    rotA.assign(static_cast<size_t>(70) * 2 * nrf * 64, 0.0);
+    coaxA.assign(static_cast<size_t>(28) * ncf * 64, 0.0);
+    for (int cls = 0; cls < 70; ++cls)
+        for (int dir = 0; dir < 2; ++dir) {
+            const double* blocks = t.rot.data() + (static_cast<size_t>(cls) * 2 + dir) * t.rot_size;
+            double* out = rotA.data() + (static_cast<size_t>(cls) * 2 + dir) * nrf * 64;
+            for (int n = 0; n <= P; ++n)
+                for (int part = 0; part < 2; ++part) {
+                    if (part == 1 && n == 0) continue;
+                    for (int mt = 0; mt < rot_mts(n); ++mt)
+                        for (int ks = 0; ks < rot_nks(n, part); ++ks) {
+                            double* f = out + static_cast<size_t>(rot_frag(n, part, mt) + ks) * 64;
+                            for (int l = 0; l < 64; ++l) {
+                                const int row = 16 * mt + l % 16, kk = 4 * ks + l / 16;
+                                double v = 0.0;
+                                if (part == 0 && row <= n && kk <= n)
+                                    v = blocks[t.rot_off[n] + row * (n + 1) + kk];
+                                else if (part == 1 && row >= 1 && row <= n && kk < n)
+                                    v = blocks[t.rot_off[n] + (n + 1) * (n + 1) + (row - 1) * n + kk];
+                                f[l] = v;
+                            }
+                        }
+                }
+        }
+    for (int d2 = 0; d2 < 28; ++d2) {
+        const double* C = t.coax.data() + static_cast<size_t>(d2) * t.coax_size;
+        double* out = coaxA.data() + static_cast<size_t>(d2) * ncf * 64;
+        for (int k = 0; k <= P; ++k) {
+            const int w = P + 1 - k;
+            for (int mt = 0; mt < coax_mts(P, k); ++mt)
+                for (int ks = 0; ks < coax_nks(P, k); ++ks) {
+                    double* f = out + static_cast<size_t>(coax_frag(P, k, mt) + ks) * 64;
+                    for (int l = 0; l < 64; ++l) {
+                        const int row = 16 * mt + l % 16, kk = 4 * ks + l / 16;
+                        f[l] = (row < w && kk < w) ? C[t.coax_off[k] + row * w + kk] : 0.0;
+                    }
+                }
+        }
+    }
 }
 
 void build_tables(int p, Tables& t)

@@ -45,6 +45,10 @@ struct Tables {
     std::vector<int32_t> om;     // om[rf index] -> order-major position
 };
 
+// MFMA A-operand fragments of the M2L blocks (layout in m2l_sched.hpp):
+// rotA [70 classes][2 directions][nfrag][64], coaxA [28 |d|^2][nfrag][64].
+void build_m2l_fragments(const Tables& t, std::vector<double>& rotA, std::vector<double>& coaxA);
+
 // Builds all tables for expansion order p (1 <= p <= 20).
 void build_tables(int p, Tables& t);
 

@@ -68,13 +68,14 @@ def _pos(kind, n, seed):
             "sphere": W.sphere_shell}[kind](n, seed=seed)[0]
 
 
+@pytest.mark.parametrize("m2l_kernel", [0, 1], ids=["mfma", "rows"])
 @pytest.mark.parametrize("kind,n,cap,p", ORACLE_CASES)
-def test_all_kinds_match_oracle(kind, n, cap, p):
-    print(f"case {kind} n={n} cap={cap} p={p}", flush=True)
+def test_all_kinds_match_oracle(kind, n, cap, p, m2l_kernel):
+    print(f"case {kind} n={n} cap={cap} p={p} m2l_kernel={m2l_kernel}", flush=True)
     pos = _pos(kind, n, 41)
     max_level = 4 if cap == 2 else 20
     part = api.OctreePartitioner(pos, api.Box(pos)).refine(max_level, cap)
-    fmm = api.FMM3D(part, p, ncharge=4)
+    fmm = api.FMM3D(part, p, ncharge=4, m2l_kernel=m2l_kernel)
     ot = O.Tree(pos, max_level, cap)
     rng = np.random.default_rng(n + p)
     q1 = rng.uniform(-1, 1, n)
@@ -141,3 +142,26 @@ def test_full_size_uniform_against_direct_sum():
     Wc = np.zeros((n, 4))
     fmm.rinv_potgrad(2.0 * q - 0.5 * q2, Wc)
     assert rel_l2(Wc, 2.0 * U - 0.5 * V) < 1e-12
+
+
+def test_mfma_and_row_kernels_agree_at_size():
+    pos, q = W.plummer(200000, seed=46)
+    part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, 64)
+    a = np.zeros((200000, 4))
+    b = np.zeros((200000, 4))
+    api.FMM3D(part, 12, ncharge=1, m2l_kernel=0).rinv_potgrad(q, a, api.FAR)
+    api.FMM3D(part, 12, ncharge=1, m2l_kernel=1).rinv_potgrad(q, b, api.FAR)
+    assert rel_l2(a, b) < 1e-13
+
+
+@pytest.mark.parametrize("p", [8, 12])
+def test_mixed_precision_close_to_fp64(p):
+    pos, q = W.plummer(100000, seed=47)
+    part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, 64)
+    a = np.zeros((100000, 4))
+    b = np.zeros((100000, 4))
+    api.FMM3D(part, p, ncharge=1).rinv_potgrad(q, a)
+    api.FMM3D(part, p, ncharge=1, precision=api.MIXED).rinv_potgrad(q, b)
+    err_phi, err_grad = rel_l2(b[:, 0], a[:, 0]), rel_l2(b[:, 1:], a[:, 1:])
+    print(f"mixed vs fp64 p={p}: phi {err_phi:.3e} grad {err_grad:.3e}", flush=True)
+    assert err_phi < 1e-6 and err_grad < 1e-6
