@@ -57,6 +57,9 @@ typedef struct {
     int threads;               /* host threads for the build (0 = hardware)           */
 } sfmm_tree_desc;
 
+/* Box(positions, vertical_axis, phideg) alone (box.hpp:84-107): out4 = width, corner x, y, z. */
+int sfmm_box_from_positions(int64_t n, const double* positions, int vertical_axis, double phideg, double out4[4]);
+
 /* Box(...) + OctreePartitioner(positions, box) + refine(max_level, cap):
  * box.hpp:73-107, partition.hpp:397-446.  The tree copies the positions. */
 int sfmm_tree_create(const sfmm_tree_desc* desc, sfmm_tree* out);

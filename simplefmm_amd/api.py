@@ -58,6 +58,7 @@ class PlanDesc(ctypes.Structure):
 
 # symbol -> (restype, argtypes); every function declared in include/sfmm.h
 SIGNATURES = {
+    "sfmm_box_from_positions": (ctypes.c_int, [ctypes.c_int64, _dp, ctypes.c_int, ctypes.c_double, _dp]),
     "sfmm_tree_create": (ctypes.c_int, [ctypes.POINTER(TreeDesc), ctypes.POINTER(ctypes.c_void_p)]),
     "sfmm_tree_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sfmm_tree_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TreeInfo)]),

@@ -64,11 +64,14 @@ struct sfmm_plan_s {
     hipEvent_t ev[9] = {};
     bool timed = false;
     int m2l_kernel = 0;
+    sfmm::P2PItems p2p;                // near-field work items
+    double* p2p_partial = nullptr;     // p2p.n_partial_rows x 4 doubles
     sfmm::M2LArgs m2l[5];              // offset-grouped M2L work per component count (built on first use)
     const double* rotA64 = nullptr;
     const double* coaxA64 = nullptr;
     const float* rotA32 = nullptr;
     const float* coaxA32 = nullptr;
+    const double* zero_cols = nullptr;  // 4 (p+1)^2 zeros: M2L columns of absent sources
     hipEvent_t* last_ev = nullptr;   // event set of the last evaluation
     std::vector<hipEvent_t> rec_ev;  // sets of 9 events, one set per recorded evaluation
     int rec_count = 0, rec_used = 0;
@@ -82,6 +85,7 @@ struct sfmm_plan_s {
         (void)hipFree(obuf);
         (void)hipFree(hQ);
         (void)hipFree(hU);
+        (void)hipFree(p2p_partial);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : rec_ev) (void)hipEventDestroy(e);
@@ -148,6 +152,10 @@ int ensure_buffers(sfmm_plan_s* P, int nc, int qdim)
             return fail(SFMM_ERR_ALLOC, "hipMalloc failed (charges)");
         P->qdim_alloc = qdim;
     }
+    if (P->p2p.n_partial_rows > 0 && !P->p2p_partial) {
+        if (hipMalloc(&P->p2p_partial, sizeof(double) * 4 * P->p2p.n_partial_rows) != hipSuccess)
+            return fail(SFMM_ERR_ALLOC, "hipMalloc failed (near-field partial sums)");
+    }
     if (nc > P->nout_alloc) {
         (void)hipFree(P->obuf);
         P->obuf = nullptr;
@@ -168,16 +176,31 @@ int build_m2l_blocks(sfmm_plan_s* P, int nc)
     const int TB = 16 / nc;
     std::vector<int32_t> blk_tgt, ent_src, src(343 * TB);
     std::vector<int64_t> blk_row{0};
-    std::vector<int16_t> ent_off;
+    std::vector<int32_t> ent_meta;
     std::vector<char> used(343, 0);
-    std::vector<int32_t> nodes;
+    // offsets in (polar class, distance class, id) order: consecutive offsets
+    // share the rotation operators, and mostly the coaxial one as well
+    const auto& cls = P->tables.off_class;
+    const auto& d2 = P->tables.off_d2;
+    std::vector<int> order;
+    for (int off = 0; off < 343; ++off)
+        if (cls[off] >= 0) order.push_back(off);
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        return cls[x] != cls[y] ? cls[x] < cls[y] : d2[x] != d2[y] ? d2[x] < d2[y] : x < y;
+    });
+    std::vector<int32_t> oct_nodes[8];
     for (int l = 2; l <= t.depth; ++l) {
         const auto& lv = t.lv[l];
-        for (int o = 0; o < 8; ++o) {
-            nodes.clear();
-            for (int32_t a = 0; a < lv.n_node(); ++a)
-                if ((lv.code[a] & 7) == o) nodes.push_back(a);
-            for (size_t b = 0; b < nodes.size(); b += TB) {
+        for (auto& v : oct_nodes) v.clear();
+        for (int32_t a = 0; a < lv.n_node(); ++a) oct_nodes[lv.code[a] & 7].push_back(a);
+        size_t nchunk = 0;
+        for (const auto& v : oct_nodes) nchunk = std::max(nchunk, (v.size() + TB - 1) / TB);
+        // the q-th chunks of the 8 octants (children of nearby parents) are adjacent
+        for (size_t q = 0; q < nchunk; ++q)
+            for (int o = 0; o < 8; ++o) {
+                const auto& nodes = oct_nodes[o];
+                const size_t b = q * TB;
+                if (b >= nodes.size()) continue;
                 std::fill(src.begin(), src.end(), -1);
                 for (int tt = 0; tt < TB; ++tt) {
                     if (b + tt >= nodes.size()) {
@@ -192,27 +215,27 @@ int build_m2l_blocks(sfmm_plan_s* P, int nc)
                         used[off] = 1;
                     }
                 }
-                for (int off = 0; off < 343; ++off) {
+                for (int off : order) {
                     if (!used[off]) continue;
                     used[off] = 0;
-                    ent_off.push_back(static_cast<int16_t>(off));
+                    ent_meta.push_back(off | cls[off] << 9 | d2[off] << 16);
                     ent_src.insert(ent_src.end(), src.begin() + off * TB, src.begin() + (off + 1) * TB);
                 }
-                blk_row.push_back(static_cast<int64_t>(ent_off.size()));
+                blk_row.push_back(static_cast<int64_t>(ent_meta.size()));
+                if (blk_row.back() - blk_row[blk_row.size() - 2] > 189)
+                    return fail(SFMM_ERR_STATE, "M2L block with more than 189 interaction offsets");
             }
-        }
     }
     A.n_block = static_cast<int64_t>(blk_row.size()) - 1;
     int rc;
     if ((rc = P->upload(blk_tgt, &A.blk_tgt)) || (rc = P->upload(blk_row, &A.blk_row)) ||
-        (rc = P->upload(ent_off, &A.ent_off)) || (rc = P->upload(ent_src, &A.ent_src)))
+        (rc = P->upload(ent_meta, &A.ent_meta)) || (rc = P->upload(ent_src, &A.ent_src)))
         return rc;
     A.fp32 = P->precision == SFMM_MIXED;
     A.rotA = A.fp32 ? static_cast<const void*>(P->rotA32) : static_cast<const void*>(P->rotA64);
     A.coaxA = A.fp32 ? static_cast<const void*>(P->coaxA32) : static_cast<const void*>(P->coaxA64);
-    A.off_class = P->d.off_class;
-    A.off_d2 = P->d.off_d2;
     A.off_cs = P->d.off_cs;
+    A.zero = P->zero_cols;
     return SFMM_OK;
 }
 
@@ -255,7 +278,8 @@ int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int
     HIP_TRY(hipEventRecord(ev[5], s));
     if (far) sfmm::launch_l2p(d, nc, P->L, P->obuf, s);
     HIP_TRY(hipEventRecord(ev[6], s));
-    if (which & SFMM_NEAR) sfmm::launch_p2p(d, mode, nc, P->qbuf, P->obuf, s);
+    if (which & SFMM_NEAR)
+        sfmm::launch_p2p(d, P->p2p, mode, nc, P->qbuf, P->obuf, P->p2p_partial, s, P->precision == SFMM_MIXED);
     HIP_TRY(hipEventRecord(ev[7], s));
     sfmm::launch_scatter_outputs(d, nc, P->obuf, dU, s);
     HIP_TRY(hipEventRecord(ev[8], s));
@@ -270,6 +294,16 @@ int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int
 extern "C" {
 
 const char* sfmm_last_error(void) { return g_last_error.c_str(); }
+
+int sfmm_box_from_positions(int64_t n, const double* positions, int vertical_axis, double phideg, double out4[4])
+{
+    if (!positions || !out4 || n <= 0) return fail(SFMM_ERR_ARG, "null argument or empty positions");
+    if (vertical_axis < 0 || vertical_axis > 2) return fail(SFMM_ERR_ARG, "vertical_axis must be 0, 1 or 2");
+    const sfmm::BoxGeom b = sfmm::make_box(n, positions, vertical_axis, phideg);
+    out4[0] = b.width;
+    for (int d = 0; d < 3; ++d) out4[1 + d] = b.corner[d];
+    return SFMM_OK;
+}
 
 int sfmm_tree_create(const sfmm_tree_desc* desc, sfmm_tree* out)
 {
@@ -444,6 +478,39 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
             il_row.push_back(static_cast<int64_t>(il_src.size()));
         }
     }
+    // ---- near-field work items: a leaf's concatenated near sources cut into runs
+    //      of at most ~2^18 pairs; split leaves go through deterministic partial sums
+    std::vector<int64_t> near_pref(t.near_begin.size());
+    std::vector<int64_t> it_leaf, it_begin, it_count, it_part, sp_slot, sp_first, sp_nitem, sp_stride;
+    int64_t prow = 0;
+    for (size_t g = 0; g < t.leaf_node.size(); ++g) {
+        int64_t total = 0;
+        for (int64_t r = t.near_row[g]; r < t.near_row[g + 1]; ++r) {
+            near_pref[r] = total;
+            total += t.near_end[r] - t.near_begin[r];
+        }
+        const int64_t nt = lf_end[g] - lf_begin[g];
+        const int64_t per = std::max<int64_t>(64, ((int64_t(1) << 18) / std::max<int64_t>(nt, 1)) / 64 * 64);
+        const int64_t nitem = std::max<int64_t>(1, (total + per - 1) / per);
+        for (int64_t i = 0; i < nitem; ++i) {
+            it_leaf.push_back(static_cast<int64_t>(g));
+            it_begin.push_back(i * per);
+            it_count.push_back(std::min(per, total - i * per));
+            it_part.push_back(nitem == 1 ? -1 : prow + i * nt);
+        }
+        if (nitem > 1) {
+            for (int64_t k = 0; k < nt; ++k) {
+                sp_slot.push_back(lf_begin[g] + k);
+                sp_first.push_back(prow + k);
+                sp_nitem.push_back(nitem);
+                sp_stride.push_back(nt);
+            }
+            prow += nitem * nt;
+        }
+    }
+    P->p2p.n_item = static_cast<int64_t>(it_leaf.size());
+    P->p2p.n_split_targets = static_cast<int64_t>(sp_slot.size());
+    P->p2p.n_partial_rows = prow;
     std::vector<double> inv_width(D + 1);
     for (int l = 0; l <= D; ++l) inv_width[l] = std::ldexp(1.0, l) / t.box.width;
     // ---- tables
@@ -491,7 +558,7 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
     UP(t.perm, perm); UP(iperm, iperm);
     UP(fl_gnode, fl_gnode); UP(fl_level, fl_level); UP(fl_begin, fl_begin); UP(fl_end, fl_end);
     UP(lf_begin, lf_begin); UP(lf_end, lf_end);
-    UP(t.near_row, near_row); UP(t.near_begin, near_begin); UP(t.near_end, near_end);
+    UP(t.near_row, near_row); UP(t.near_begin, near_begin); UP(t.near_end, near_end); UP(near_pref, near_pref);
     UP(octant, octant); UP(parent, parent); UP(child0, child0); UP(nchild, nchild);
     UP(tgt, tgt_gnode); UP(il_row, il_row); UP(il_src, il_src); UP(il_off, il_off);
     UP(inv_width, inv_width);
@@ -499,6 +566,11 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
     UP(tb.off_class, off_class); UP(tb.off_d2, off_d2); UP(tb.om, om); UP(om_rows, om_rows);
     UP(tb.rot_off, rot_off); UP(tb.coax_off, coax_off); UP(rec, rec);
 #undef UP
+    if ((rc = P->upload(it_leaf, &P->p2p.leaf)) || (rc = P->upload(it_begin, &P->p2p.src_begin)) ||
+        (rc = P->upload(it_count, &P->p2p.src_count)) || (rc = P->upload(it_part, &P->p2p.partial)) ||
+        (rc = P->upload(sp_slot, &P->p2p.split_slot)) || (rc = P->upload(sp_first, &P->p2p.split_first)) ||
+        (rc = P->upload(sp_nitem, &P->p2p.split_nitem)) || (rc = P->upload(sp_stride, &P->p2p.split_stride)))
+        return rc;
     {
         std::vector<double> rotA, coaxA;
         sfmm::build_m2l_fragments(tb, rotA, coaxA);
@@ -508,6 +580,7 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
         } else {
             if ((rc = P->upload(rotA, &P->rotA64)) || (rc = P->upload(coaxA, &P->coaxA64))) return rc;
         }
+        if ((rc = P->upload(std::vector<double>(4 * P->K, 0.0), &P->zero_cols))) return rc;
     }
     *out = P.release();
     return SFMM_OK;

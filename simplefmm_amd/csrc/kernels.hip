@@ -10,6 +10,8 @@
 // rho^n Z(n,m,theta) e^{-i m phi} (fmm.hpp:107-111,226-232,537-548).
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace sfmm {
 
 namespace {
@@ -479,54 +481,107 @@ __device__ __forceinline__ void pair_kernel(double dx, double dy, double dz, con
     }
 }
 
-template <int MODE, int NCQ, int NOUT>
-__global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, const double* __restrict__ q, double* __restrict__ out)
+// Mixed-precision near field: coordinates are taken relative to the target
+// leaf's first particle in FP64 and rounded to FP32 once per staged source, the
+// pair arithmetic is FP32 (v_rsq_f32), and each target's sum over a source tile
+// of 64 is added to an FP64 accumulator.
+template <int MODE, int NCQ>
+__device__ __forceinline__ void pair_kernel_f32(float dx, float dy, float dz, const float* qj, float* acc)
 {
-    __shared__ double sx[64], sy[64], sz[64], sq[NCQ][64];
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float rinv = rsqrtf(r2);
+    if constexpr (MODE == MODE_POT) {
+#pragma unroll
+        for (int c = 0; c < NCQ; ++c) acc[c] += qj[c] * rinv;
+    } else if constexpr (MODE == MODE_CURL) {
+        const float r3 = rinv * rinv * rinv;
+        acc[0] += (qj[1] * dz - qj[2] * dy) * r3;
+        acc[1] += (qj[2] * dx - qj[0] * dz) * r3;
+        acc[2] += (qj[0] * dy - qj[1] * dx) * r3;
+    } else {
+        const float qr = qj[0] * rinv;
+        const float qr3 = qr * rinv * rinv;
+        acc[0] += qr;
+        acc[1] -= qr3 * dx;
+        acc[2] -= qr3 * dy;
+        acc[3] -= qr3 * dz;
+    }
+}
+
+// Near field (N2N, fmm.hpp:773-809), one wavefront per work item.  An item is
+// a target leaf and a run of at most a few thousand sources of its near list
+// (the concatenation of its merged source ranges): leaves whose near list is
+// long (coarse leaves next to deep subtrees) are split over several items so
+// that no wavefront carries a long tail.  Lanes are (target, split): a leaf of
+// nt < 64 targets spreads each target over floor(64/nt) lanes.  Sources are
+// staged through LDS 64 at a time.  An unsplit item adds into the output; the
+// items of a split leaf write partial sums that p2p_reduce_kernel adds in a
+// fixed order, so results are bitwise repeatable.
+// MIXED: coordinates relative to the target leaf's first particle, rounded to
+// FP32 once per staged source; FP32 pair arithmetic; FP64 accumulation per tile.
+template <int MODE, int NCQ, int NOUT, bool MIXED>
+__global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, P2PItems w, const double* __restrict__ q,
+                                                 double* __restrict__ out, double* __restrict__ partial)
+{
+    using R = typename std::conditional<MIXED, float, double>::type;
+    __shared__ R sx[64], sy[64], sz[64], sq[NCQ][64];
+    __shared__ int64_t sidx[64];
     __shared__ double red[NOUT][64];
-    const int64_t g = blockIdx.x;
+    const int64_t it = blockIdx.x;
     const int tid = threadIdx.x;
+    const int64_t g = w.leaf[it];
+    const int64_t u0 = w.src_begin[it], u1 = u0 + w.src_count[it];
+    const int64_t pslot = w.partial[it];
     const int64_t tb = d.lf_begin[g], te = d.lf_end[g];
     const int64_t nt = te - tb;
-    int npow = 1;
-    while (npow < nt && npow < 64) npow <<= 1;
-    const int S = 64 / npow;
-    const int t = tid % npow, s = tid / npow;
-    for (int64_t t0 = tb; t0 < te; t0 += npow) {
+    const int64_t r0 = d.near_row[g], r1 = d.near_row[g + 1];
+    const double ox = MIXED ? d.px[tb] : 0.0, oy = MIXED ? d.py[tb] : 0.0, oz = MIXED ? d.pz[tb] : 0.0;
+    const int ntl = nt < 64 ? static_cast<int>(nt) : 64;  // targets per pass
+    const int S = 64 / ntl;
+    const int t = tid % ntl, s = tid / ntl;
+    const bool lane_on = s < S;
+    for (int64_t t0 = tb; t0 < te; t0 += ntl) {
         const int64_t slot = t0 + t;
-        const bool valid = slot < te;
-        double xi = 0, yi = 0, zi = 0;
+        const bool valid = lane_on && slot < te;
+        R xi = 0, yi = 0, zi = 0;
         if (valid) {
-            xi = d.px[slot];
-            yi = d.py[slot];
-            zi = d.pz[slot];
+            xi = static_cast<R>(d.px[slot] - ox);
+            yi = static_cast<R>(d.py[slot] - oy);
+            zi = static_cast<R>(d.pz[slot] - oz);
         }
         double acc[NOUT];
 #pragma unroll
         for (int c = 0; c < NOUT; ++c) acc[c] = 0.0;
-        for (int64_t r = d.near_row[g]; r < d.near_row[g + 1]; ++r) {
-            const int64_t rb = d.near_begin[r], re = d.near_end[r];
-            for (int64_t j0 = rb; j0 < re; j0 += 64) {
-                __syncthreads();
-                const int64_t jl = j0 + tid;
-                if (jl < re) {
-                    sx[tid] = d.px[jl];
-                    sy[tid] = d.py[jl];
-                    sz[tid] = d.pz[jl];
+        int64_t rc = r0;  // range holding source u of this lane, advanced monotonically
+        for (int64_t ut = u0; ut < u1; ut += 64) {
+            __syncthreads();
+            const int64_t u = ut + tid;
+            if (u < u1) {
+                while (rc + 1 < r1 && d.near_pref[rc + 1] <= u) ++rc;
+                const int64_t j = d.near_begin[rc] + (u - d.near_pref[rc]);
+                sidx[tid] = j;
+                sx[tid] = static_cast<R>(d.px[j] - ox);
+                sy[tid] = static_cast<R>(d.py[j] - oy);
+                sz[tid] = static_cast<R>(d.pz[j] - oz);
 #pragma unroll
-                    for (int c = 0; c < NCQ; ++c) sq[c][tid] = q[c * d.n + jl];
-                }
-                __syncthreads();
-                const int cnt = static_cast<int>(re - j0 < 64 ? re - j0 : 64);
-                if (valid) {
-                    for (int k = s; k < cnt; k += S) {
-                        if (j0 + k == slot) continue;
-                        double qj[NCQ];
+                for (int c = 0; c < NCQ; ++c) sq[c][tid] = static_cast<R>(q[c * d.n + j]);
+            }
+            __syncthreads();
+            const int cnt = static_cast<int>(u1 - ut < 64 ? u1 - ut : 64);
+            if (valid) {
+                R a[NOUT];
 #pragma unroll
-                        for (int c = 0; c < NCQ; ++c) qj[c] = sq[c][k];
-                        pair_kernel<MODE, NCQ>(xi - sx[k], yi - sy[k], zi - sz[k], qj, acc);
-                    }
+                for (int c = 0; c < NOUT; ++c) a[c] = 0;
+                for (int k = s; k < cnt; k += S) {
+                    if (sidx[k] == slot) continue;
+                    R qj[NCQ];
+#pragma unroll
+                    for (int c = 0; c < NCQ; ++c) qj[c] = sq[c][k];
+                    if constexpr (MIXED) pair_kernel_f32<MODE, NCQ>(xi - sx[k], yi - sy[k], zi - sz[k], qj, a);
+                    else pair_kernel<MODE, NCQ>(xi - sx[k], yi - sy[k], zi - sz[k], qj, a);
                 }
+#pragma unroll
+                for (int c = 0; c < NOUT; ++c) acc[c] += static_cast<double>(a[c]);
             }
         }
         __syncthreads();
@@ -537,10 +592,26 @@ __global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, const double* __rest
 #pragma unroll
             for (int c = 0; c < NOUT; ++c) {
                 double v = 0.0;
-                for (int ss = 0; ss < S; ++ss) v += red[c][t + ss * npow];
-                out[c * d.n + slot] += v;
+                for (int ss = 0; ss < S; ++ss) v += red[c][t + ss * ntl];
+                if (pslot < 0) out[c * d.n + slot] += v;
+                else partial[(pslot + (slot - tb)) * NOUT + c] = v;
             }
         }
+    }
+}
+
+// Adds the partial sums of split leaves in item order.
+__global__ void p2p_reduce_kernel(DevPlan d, P2PItems w, int nout, const double* __restrict__ partial,
+                                  double* __restrict__ out)
+{
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= w.n_split_targets) return;
+    const int64_t slot = w.split_slot[k];
+    const int64_t first = w.split_first[k], nitem = w.split_nitem[k], stride = w.split_stride[k];
+    for (int c = 0; c < nout; ++c) {
+        double v = 0.0;
+        for (int64_t i = 0; i < nitem; ++i) v += partial[(first + i * stride) * nout + c];
+        out[c * d.n + slot] += v;
     }
 }
 
@@ -697,22 +768,37 @@ void launch_l2l(const DevPlan& d, LevelSpan children, int nc, double* L, hipStre
                        sizeof(double) * nc * d.K, s, d, children.first, nc, L);
 }
 
-void launch_p2p(const DevPlan& d, Mode mode, int nc, const double* q, double* out, hipStream_t s)
+template <bool MIXED>
+void launch_p2p_t(const DevPlan& d, const P2PItems& w, Mode mode, int nc, const double* q, double* out,
+                  double* partial, hipStream_t s)
 {
-    if (d.n_leaf == 0) return;
-    const dim3 grid(static_cast<unsigned>(d.n_leaf)), blk(64);
+    const dim3 grid(static_cast<unsigned>(w.n_item)), blk(64);
+    int nout = nc;
     if (mode == MODE_POTGRAD) {
-        hipLaunchKernelGGL((p2p_kernel<MODE_POTGRAD, 1, 4>), grid, blk, 0, s, d, q, out);
+        hipLaunchKernelGGL((p2p_kernel<MODE_POTGRAD, 1, 4, MIXED>), grid, blk, 0, s, d, w, q, out, partial);
+        nout = 4;
     } else if (mode == MODE_CURL) {
-        hipLaunchKernelGGL((p2p_kernel<MODE_CURL, 3, 3>), grid, blk, 0, s, d, q, out);
+        hipLaunchKernelGGL((p2p_kernel<MODE_CURL, 3, 3, MIXED>), grid, blk, 0, s, d, w, q, out, partial);
+        nout = 3;
     } else {
         switch (nc) {
-        case 1: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 1, 1>), grid, blk, 0, s, d, q, out); break;
-        case 2: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 2, 2>), grid, blk, 0, s, d, q, out); break;
-        case 3: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 3, 3>), grid, blk, 0, s, d, q, out); break;
-        default: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 4, 4>), grid, blk, 0, s, d, q, out); break;
+        case 1: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 1, 1, MIXED>), grid, blk, 0, s, d, w, q, out, partial); break;
+        case 2: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 2, 2, MIXED>), grid, blk, 0, s, d, w, q, out, partial); break;
+        case 3: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 3, 3, MIXED>), grid, blk, 0, s, d, w, q, out, partial); break;
+        default: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 4, 4, MIXED>), grid, blk, 0, s, d, w, q, out, partial); break;
         }
     }
+    if (w.n_split_targets > 0)
+        hipLaunchKernelGGL(p2p_reduce_kernel, dim3(grid_for(w.n_split_targets, 256)), dim3(256), 0, s, d, w, nout,
+                           partial, out);
+}
+
+void launch_p2p(const DevPlan& d, const P2PItems& w, Mode mode, int nc, const double* q, double* out,
+                double* partial, hipStream_t s, bool mixed)
+{
+    if (w.n_item == 0) return;
+    if (mixed) launch_p2p_t<true>(d, w, mode, nc, q, out, partial, s);
+    else launch_p2p_t<false>(d, w, mode, nc, q, out, partial, s);
 }
 
 int64_t direct_scratch_size(int64_t ntarget, int nout, int nsplit) { return ntarget * nout * nsplit; }

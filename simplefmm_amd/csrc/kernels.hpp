@@ -32,6 +32,7 @@ struct DevPlan {
     int64_t n_leaf = 0;
     const int64_t *lf_begin = nullptr, *lf_end = nullptr;
     const int64_t *near_row = nullptr, *near_begin = nullptr, *near_end = nullptr;
+    const int64_t* near_pref = nullptr;  // per range: sources of the leaf's earlier ranges
     // nodes
     const int8_t* octant = nullptr;     // code & 7 per gnode
     const int32_t* parent = nullptr;    // parent gnode (-1 at level 0)
@@ -59,14 +60,29 @@ struct M2LArgs {
     int64_t n_block = 0;
     const int32_t* blk_tgt = nullptr;  // [n_block][16/nc] target gnodes, -1 = none
     const int64_t* blk_row = nullptr;  // [n_block+1] entry ranges
-    const int16_t* ent_off = nullptr;  // offset id per entry
+    const int32_t* ent_meta = nullptr; // per entry: offset id | polar class << 9 | distance class << 16
     const int32_t* ent_src = nullptr;  // [entries][16/nc] source gnodes, -1 = none
     const void* rotA = nullptr;        // rotation fragments [70][2][nfrag][64], double or float
     const void* coaxA = nullptr;       // coaxial fragments [28][nfrag][64]
-    const int32_t* off_class = nullptr;
-    const int32_t* off_d2 = nullptr;
-    const double* off_cs = nullptr;
+    const double* off_cs = nullptr;    // [343][p+1] (cos, sin) of m * azimuth
+    const double* zero = nullptr;      // nc * (p+1)^2 zeros: the columns of absent sources
     int fp32 = 0;                      // 1: FP32 MFMA stages (mixed precision)
+};
+
+// Near-field work: items of (leaf, run of its concatenated near sources).
+struct P2PItems {
+    int64_t n_item = 0;
+    const int64_t* leaf = nullptr;       // global leaf id
+    const int64_t* src_begin = nullptr;  // first source (index into the concatenated near list)
+    const int64_t* src_count = nullptr;
+    const int64_t* partial = nullptr;    // -1: add into the output; else first partial row
+    // split leaves: per target, the item partial rows first + i*stride, i < nitem
+    int64_t n_split_targets = 0;
+    int64_t n_partial_rows = 0;
+    const int64_t* split_slot = nullptr;
+    const int64_t* split_first = nullptr;
+    const int64_t* split_nitem = nullptr;
+    const int64_t* split_stride = nullptr;
 };
 
 struct LevelSpan {
@@ -84,7 +100,10 @@ void launch_l2p(const DevPlan& d, int nc, const double* L, double* out, hipStrea
 void launch_m2l_mfma(const M2LArgs& a, int p, int nc, const double* M, double* L, hipStream_t s);
 int m2l_rot_nfrag(int p);
 int m2l_coax_nfrag(int p);
-void launch_p2p(const DevPlan& d, Mode mode, int nc, const double* q, double* out, hipStream_t s);
+// mixed: FP32 pair arithmetic with FP64 accumulation per source tile.
+// partial: P2PItems::n_partial_rows x nout doubles of scratch.
+void launch_p2p(const DevPlan& d, const P2PItems& w, Mode mode, int nc, const double* q, double* out,
+                double* partial, hipStream_t s, bool mixed);
 void launch_direct(const DevPlan& d, Mode mode, int nc, const double* q, int64_t ntarget, const int64_t* targets,
                    double* out, double* partial, int nsplit, hipStream_t s);
 void launch_fill(double* x, int64_t count, double v, hipStream_t s);

@@ -56,23 +56,24 @@ __host__ __device__ constexpr int coax_frag(int P, int k, int mt)
 }
 __host__ __device__ constexpr int coax_nfrag(int P) { return coax_frag(P, P + 1, 0); }
 
-// Static work split over the 4 wavefronts of a workgroup: longest tiles first,
+// Static work split over the NW wavefronts of a workgroup: longest tiles first,
 // each to the least loaded wave.
+constexpr int kMaxWaves = 8;
 struct WaveTiles {
-    int count[4] = {0, 0, 0, 0};
-    int item[4][48] = {};  // packed (index << 8) | (part << 4) | mt
+    int count[kMaxWaves] = {};
+    int item[kMaxWaves][48] = {};  // packed (index << 8) | (part << 4) | mt
 };
 
-template <int P>
+template <int P, int NW>
 struct M2LSchedule {
     WaveTiles s1, s2, s3;  // s3 items are degrees j (both parts, all m-tiles)
 
     constexpr M2LSchedule()
     {
-        int load[4] = {0, 0, 0, 0};
+        int load[kMaxWaves] = {};
         auto pick = [&](int cost) {
             int w = 0;
-            for (int v = 1; v < 4; ++v)
+            for (int v = 1; v < NW; ++v)
                 if (load[v] < load[w]) w = v;
             load[w] += cost;
             return w;
@@ -85,7 +86,7 @@ struct M2LSchedule {
                     s1.item[w][s1.count[w]++] = (n << 8) | (part << 4) | mt;
                 }
             }
-        for (int v = 0; v < 4; ++v) load[v] = 0;
+        for (int v = 0; v < kMaxWaves; ++v) load[v] = 0;
         for (int k = 0; k <= P; ++k)
             for (int part = 0; part < 2; ++part) {
                 if (part == 1 && k == 0) continue;
@@ -94,7 +95,7 @@ struct M2LSchedule {
                     s2.item[w][s2.count[w]++] = (k << 8) | (part << 4) | mt;
                 }
             }
-        for (int v = 0; v < 4; ++v) load[v] = 0;
+        for (int v = 0; v < kMaxWaves; ++v) load[v] = 0;
         for (int j = P; j >= 0; --j) {
             const int w = pick(rot_mts(j) * (rot_nks(j, 0) + rot_nks(j, 1)));
             s3.item[w][s3.count[w]++] = j << 8;
