@@ -158,220 +158,511 @@ __device__ __forceinline__ int ent_offset(int w) { return w & 511; }
 __device__ __forceinline__ int ent_class(int w) { return (w >> 9) & 127; }
 __device__ __forceinline__ int ent_dist(int w) { return w >> 16; }
 
-// One wavefront's part of a block.  Per entry (offset): gathered columns into
-// X (LDS), stage 1 X -> Y, stage 2 Y -> X, stage 3 from X into FP64 registers.
-// Every global load an entry needs (source columns, azimuth cos/sin, operators
-// when the polar or distance class changes) is issued at the top of the
-// previous entry, with no branch between the load and the end of that entry,
-// so its latency overlaps a whole entry of matrix work.  The entry list of the
-// block sits in LDS.
-template <typename T, int P, int NC, int W>
-__device__ __forceinline__ void m2l_wave(const M2LArgs& a, int64_t blk, int ne, const double* __restrict__ M,
-                                         double* __restrict__ L, T* Xs, T* Ys, T* cs_s, const int* ent_w,
-                                         const int* ent_s)
+// ---------------------------------------------------------------- stages
+// Each wave runs its share (schedule<P>, wave W) of a stage over the 16
+// columns held in LDS.  A stage first reads every B operand it needs, then runs
+// its MFMA chains, then writes; rows that fall outside the expansion go to the
+// trash row (kTrashRow) instead of being masked, so the whole stage is one
+// basic block the compiler can schedule freely.
+constexpr int kTrashRow = 0;  // relative to the end of the padded rows
+
+// B operands of stage 1 / 2 / 3 for wave W, in the same k-order as the A registers.
+template <int P, int W>
+__host__ __device__ constexpr int s3_boff(int q)
+{
+    const auto S = schedule<P>();
+    int o = 0;
+    for (int i = 0; i < q; ++i) {
+        const int j = S.s3.item[W][i] >> 8;
+        o += rot_nks(j, 0) + rot_nks(j, 1);
+    }
+    return o;
+}
+
+template <typename T, int P, int W>
+struct StageIn {
+    static constexpr auto S = schedule<P>();
+    static constexpr int N1 = s1_koff<P, W>(S.s1.count[W]);
+    static constexpr int N2 = s2_koff<P, W>(S.s2.count[W]);
+    static constexpr int N3 = s3_boff<P, W>(S.s3.count[W]);
+    T b1[N1 > 0 ? N1 : 1], b2[N2 > 0 ? N2 : 1], b3[N3 > 0 ? N3 : 1];
+};
+
+template <typename T, int P, int W>
+__device__ __forceinline__ void read_rot_fwd(T* b, const T* X, int lane)
+{
+    constexpr auto S = schedule<P>();
+    const int col = lane & 15;
+    static_for<0, S.s1.count[W]>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int item = S.s1.item[W][q];
+        constexpr int n = item >> 8, part = (item >> 4) & 15;
+        constexpr int brow = part == 0 ? n * n : n * n + n + 1;
+        constexpr int ko = s1_koff<P, W>(q);
+#pragma unroll
+        for (int ks = 0; ks < rot_nks(n, part); ++ks) b[ko + ks] = X[(brow + 4 * ks + (lane >> 4)) * kLdsW + col];
+    });
+}
+
+template <typename T, int P, int W>
+__device__ __forceinline__ void read_coax(T* b, const T* Y, int lane)
+{
+    constexpr auto S = schedule<P>();
+    const int col = lane & 15;
+    static_for<0, S.s2.count[W]>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int item = S.s2.item[W][q];
+        constexpr int k = item >> 8, part = (item >> 4) & 15;
+        constexpr int brow = om_row(P, k, k, part);
+        constexpr int ko = s2_koff<P, W>(q);
+#pragma unroll
+        for (int ks = 0; ks < coax_nks(P, k); ++ks) b[ko + ks] = Y[(brow + 4 * ks + (lane >> 4)) * kLdsW + col];
+    });
+}
+
+template <typename T, int P, int W>
+__device__ __forceinline__ void read_rot_back(T* b, const T* X, int lane)
+{
+    constexpr auto S = schedule<P>();
+    const int col = lane & 15;
+    static_for<0, S.s3.count[W]>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int j = S.s3.item[W][q] >> 8;
+        constexpr int bo = s3_boff<P, W>(q);
+        constexpr int nkc = rot_nks(j, 0);
+#pragma unroll
+        for (int ks = 0; ks < nkc; ++ks) b[bo + ks] = X[(j * j + 4 * ks + (lane >> 4)) * kLdsW + col];
+#pragma unroll
+        for (int ks = 0; ks < rot_nks(j, 1); ++ks)
+            b[bo + nkc + ks] = X[(j * j + j + 1 + 4 * ks + (lane >> 4)) * kLdsW + col];
+    });
+}
+
+// Longest k-chain among a wave's tiles of each stage.
+template <int P, int W>
+__host__ __device__ constexpr int s1_maxks()
+{
+    const auto S = schedule<P>();
+    int m = 0;
+    for (int q = 0; q < S.s1.count[W]; ++q) {
+        const int it = S.s1.item[W][q];
+        m = rot_nks(it >> 8, (it >> 4) & 15) > m ? rot_nks(it >> 8, (it >> 4) & 15) : m;
+    }
+    return m;
+}
+template <int P, int W>
+__host__ __device__ constexpr int s2_maxks()
+{
+    const auto S = schedule<P>();
+    int m = 0;
+    for (int q = 0; q < S.s2.count[W]; ++q) m = coax_nks(P, S.s2.item[W][q] >> 8) > m ? coax_nks(P, S.s2.item[W][q] >> 8) : m;
+    return m;
+}
+template <int P, int W>
+__host__ __device__ constexpr int s3_maxks()
+{
+    const auto S = schedule<P>();
+    int m = 0;
+    for (int q = 0; q < S.s3.count[W]; ++q) m = rot_nks(S.s3.item[W][q] >> 8, 0) > m ? rot_nks(S.s3.item[W][q] >> 8, 0) : m;
+    return m;
+}
+
+// The MFMA chains of a stage run k-step by k-step across its tiles, so that
+// consecutive MFMAs are independent.
+
+// Stage 1: polar rotation per degree, degree-major X -> order-major Y.
+template <typename T, int P, int W>
+__device__ __forceinline__ void stage_rot_fwd(const T* a1, const T* b, T* Y, int lane)
 {
     using F = Mfma<T>;
     using V4 = typename F::V4;
-    constexpr int K = (P + 1) * (P + 1);
-    constexpr int NT = (P + 1) * (P + 2) / 2;
-    constexpr int TB = 16 / NC;
-    constexpr int MT = rot_mts(P);
-    constexpr int NRF = rot_nfrag(P);
-    constexpr int NCF = coax_nfrag(P);
     constexpr auto S = schedule<P>();
-    constexpr int NJ = S.s3.count[W];
-    constexpr int NK1 = s1_koff<P, W>(S.s1.count[W]);
-    constexpr int NK2 = s2_koff<P, W>(S.s2.count[W]);
-    constexpr int NK3 = s3_koff<P, W>(NJ);
-    constexpr int CPT = kThreads / 16;        // threads per gathered column
-    constexpr int GI = (NT + CPT - 1) / CPT;  // gather items per thread
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
+    constexpr int K = (P + 1) * (P + 1);
+    constexpr int NQ = S.s1.count[W];
     const int col = lane & 15;
-    const T* rotA = static_cast<const T*>(a.rotA);
-    const T* coaxA = static_cast<const T*>(a.coaxA);
+    V4 d[NQ > 0 ? NQ : 1];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) d[q] = V4{0, 0, 0, 0};
+    static_for<0, s1_maxks<P, W>()>([&](auto kc) {
+        constexpr int ks = decltype(kc)::value;
+        static_for<0, NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int item = S.s1.item[W][q];
+            constexpr int ko = s1_koff<P, W>(q);
+            if constexpr (ks < rot_nks(item >> 8, (item >> 4) & 15)) d[q] = F::run(a1[ko + ks], b[ko + ks], d[q]);
+        });
+    });
+    static_for<0, NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int item = S.s1.item[W][q];
+        constexpr int n = item >> 8, part = (item >> 4) & 15, mt = item & 15;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * mt + F::drow(lane, r);
+            const bool ok = row <= n && (part == 0 || row > 0);
+            const int dst = ok ? om_row(P, row, n, part) : K + kPadRows + kTrashRow;
+            Y[dst * kLdsW + col] = d[q][r];
+        }
+    });
+}
 
-    double acc[NJ > 0 ? NJ : 1][MT][2][4];
+// Stage 2: coaxial translation per order, order-major Y -> degree-major X.
+template <typename T, int P, int W>
+__device__ __forceinline__ void stage_coax(const T* a2, const T* b, T* X, int lane)
+{
+    using F = Mfma<T>;
+    using V4 = typename F::V4;
+    constexpr auto S = schedule<P>();
+    constexpr int K = (P + 1) * (P + 1);
+    constexpr int NQ = S.s2.count[W];
+    const int col = lane & 15;
+    V4 d[NQ > 0 ? NQ : 1];
 #pragma unroll
-    for (int q = 0; q < (NJ > 0 ? NJ : 1); ++q)
+    for (int q = 0; q < NQ; ++q) d[q] = V4{0, 0, 0, 0};
+    static_for<0, s2_maxks<P, W>()>([&](auto kc) {
+        constexpr int ks = decltype(kc)::value;
+        static_for<0, NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int ko = s2_koff<P, W>(q);
+            if constexpr (ks < coax_nks(P, S.s2.item[W][q] >> 8)) d[q] = F::run(a2[ko + ks], b[ko + ks], d[q]);
+        });
+    });
+    static_for<0, NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int item = S.s2.item[W][q];
+        constexpr int k = item >> 8, part = (item >> 4) & 15, mt = item & 15;
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) acc[q][mt][0][r] = acc[q][mt][1][r] = 0.0;
+        for (int r = 0; r < 4; ++r) {
+            const int j = k + 16 * mt + F::drow(lane, r);
+            const int dst = j <= P ? ps_row(j, k, part) : K + kPadRows + kTrashRow;
+            X[dst * kLdsW + col] = d[q][r];
+        }
+    });
+}
 
-    // gather role: column gc = (target gt, component gcc); coefficients pr = gl + CPT*g
-    const int gc = tid / CPT, gl = tid - (tid / CPT) * CPT;
-    const int gt = gc / NC, gcc = gc - (gc / NC) * NC;
-    const bool gon = gt < TB;
-    const int gts = gon ? gt : 0;
-    int gnm[GI];  // packed n | m << 8 (always a valid coefficient); gv: written
-    bool gv[GI];
+// Stage 3: back rotation per degree, azimuth e^{-i m phi} from cs, FP64
+// accumulation.  Rows beyond a degree have all-zero operator rows, so their
+// results are zero and add nothing (cs is read at a clamped, valid index).
+template <typename T, int P, int W, class Acc>
+__device__ __forceinline__ void stage_rot_back(const T* a3, const T* b, const T* cs, Acc& acc, int lane)
+{
+    using F = Mfma<T>;
+    using V4 = typename F::V4;
+    constexpr auto S = schedule<P>();
+    constexpr int NQ = S.s3.count[W];
+    constexpr int MT = rot_mts(P);
+    V4 dc[NQ > 0 ? NQ : 1][MT], ds[NQ > 0 ? NQ : 1][MT];
 #pragma unroll
-    for (int g = 0; g < GI; ++g) {
-        const int pr = gl + CPT * g;
-        int n = 0;
-        while ((n + 1) * (n + 2) / 2 <= pr) ++n;
-        gv[g] = gon && pr < NT;
-        gnm[g] = gv[g] ? (n | ((pr - n * (n + 1) / 2) << 8)) : 0;
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) dc[q][mt] = ds[q][mt] = V4{0, 0, 0, 0};
+    static_for<0, s3_maxks<P, W>()>([&](auto kc) {
+        constexpr int ks = decltype(kc)::value;
+        static_for<0, NQ>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int j = S.s3.item[W][q] >> 8;
+            constexpr int ko = s3_koff<P, W>(q);
+            constexpr int bo = s3_boff<P, W>(q);
+            constexpr int nkc = rot_nks(j, 0), nks = rot_nks(j, 1);
+            static_for<0, rot_mts(j)>([&](auto mc) {
+                constexpr int mt = decltype(mc)::value;
+                constexpr int kb = ko + mt * (nkc + nks);
+                if constexpr (ks < nkc) dc[q][mt] = F::run(a3[kb + ks], b[bo + ks], dc[q][mt]);
+                if constexpr (ks < nks) ds[q][mt] = F::run(a3[kb + nkc + ks], b[bo + nkc + ks], ds[q][mt]);
+            });
+        });
+    });
+    static_for<0, NQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int j = S.s3.item[W][q] >> 8;
+#pragma unroll
+        for (int mt = 0; mt < rot_mts(j); ++mt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * mt + F::drow(lane, r);
+                const int rc = row <= j ? row : j;
+                const double cm = cs[2 * rc], sm = cs[2 * rc + 1];
+                const double lc = dc[q][mt][r], ls = ds[q][mt][r];
+                acc[q][mt][0][r] += lc * cm + ls * sm;
+                acc[q][mt][1][r] += ls * cm - lc * sm;
+            }
+    });
+}
+
+// Source columns of one entry as a thread gathers them: for each of its
+// items (n, m) the pair (Re, Im) of coefficient (n, m), or (c(n,0), next) for m = 0.
+template <int GI>
+struct Cols {
+    double x[GI], y[GI];
+};
+
+// Gather role and operator registers of a wave.
+template <typename T, int P, int NC, int W>
+struct WaveState {
+    static constexpr int K = (P + 1) * (P + 1);
+    static constexpr int NT = (P + 1) * (P + 2) / 2;
+    static constexpr int TB = 16 / NC;
+    static constexpr int NRF = rot_nfrag(P);
+    static constexpr int NCF = coax_nfrag(P);
+    static constexpr auto S = schedule<P>();
+    static constexpr int NJ = S.s3.count[W];
+    static constexpr int NK1 = s1_koff<P, W>(S.s1.count[W]);
+    static constexpr int NK2 = s2_koff<P, W>(S.s2.count[W]);
+    static constexpr int NK3 = s3_koff<P, W>(NJ);
+    static constexpr int CPT = kThreads / 16;        // threads per gathered column
+    static constexpr int GI = (NT + CPT - 1) / CPT;  // gather items per thread
+
+    int gc, gcc, gts;
+    int gnm[GI];  // packed n | m << 8 (always a valid coefficient)
+    bool gv[GI];  // item written
+    T a1[NK1 > 0 ? NK1 : 1], a2[NK2 > 0 ? NK2 : 1], a3[NK3 > 0 ? NK3 : 1];  // operators in use
+    T b1[NK1 > 0 ? NK1 : 1], b2[NK2 > 0 ? NK2 : 1], b3[NK3 > 0 ? NK3 : 1];  // operators loading
+
+    __device__ __forceinline__ void init(int tid)
+    {
+        gc = tid / CPT;
+        const int gl = tid - gc * CPT;
+        const int gt = gc / NC;
+        gcc = gc - gt * NC;
+        const bool gon = gt < TB;
+        gts = gon ? gt : 0;
+#pragma unroll
+        for (int g = 0; g < GI; ++g) {
+            const int pr = gl + CPT * g;
+            int n = 0;
+            while ((n + 1) * (n + 2) / 2 <= pr) ++n;
+            gv[g] = gon && pr < NT;
+            gnm[g] = gv[g] ? (n | ((pr - n * (n + 1) / 2) << 8)) : 0;
+        }
     }
-    const double2* cs2 = reinterpret_cast<const double2*>(a.off_cs);
-    const int tcs = tid < 2 * (P + 1) ? tid : 0;
 
-    double gx[GI], gy[GI], gcm[GI], gsm[GI], c3;
-    T a1[NK1 > 0 ? NK1 : 1], a2[NK2 > 0 ? NK2 : 1], a3[NK3 > 0 ? NK3 : 1];
-    T b1[NK1 > 0 ? NK1 : 1], b2[NK2 > 0 ? NK2 : 1], b3[NK3 > 0 ? NK3 : 1];
-
-    // loads of entry i (all unconditional except the operator sets)
-    auto issue = [&](int i, int w_prev, bool first) {
-        const int w = ent_w[i];
-        const int off = ent_offset(w);
-        const int32_t src = ent_s[i * TB + gts];
+    // Columns of the entry with source row srow: one 16-byte load per item
+    // (the real-form pair of (n, m) is adjacent; for m = 0 the second value is
+    // the next coefficient, or the zero padding after the last one).
+    __device__ __forceinline__ void load_cols(Cols<GI>& c, const M2LArgs& a, const double* __restrict__ M,
+                                              const int* srow) const
+    {
+        struct alignas(8) D2 {
+            double x, y;
+        };
+        const int32_t src = srow[gts];
         const double* Ms = src < 0 ? a.zero + gcc * K : M + (static_cast<int64_t>(src) * NC + gcc) * K;
 #pragma unroll
         for (int g = 0; g < GI; ++g) {
             const int n = gnm[g] & 0xff, m = gnm[g] >> 8;
-            gx[g] = Ms[rfc2(n, m)];
-            gy[g] = Ms[m > 0 ? rfs2(n, m) : rfc2(n, m)];
-            const double2 cs = cs2[off * (P + 1) + m];
-            gcm[g] = cs.x;
-            gsm[g] = cs.y;
+            const D2 v = *reinterpret_cast<const D2*>(Ms + rfc2(n, m));
+            c.x[g] = v.x;
+            c.y[g] = v.y;
         }
-        c3 = a.off_cs[off * (P + 1) * 2 + tcs];
-        const int cls = __builtin_amdgcn_readfirstlane(ent_class(w));
-        const int d2 = __builtin_amdgcn_readfirstlane(ent_dist(w));
-        if (first || cls != ent_class(w_prev)) {
-            load_rot_fwd<T, P, W>(b1, rotA + (static_cast<int64_t>(cls) * 2 + 0) * NRF * 64, lane);
-            load_rot_bwd<T, P, W>(b3, rotA + (static_cast<int64_t>(cls) * 2 + 1) * NRF * 64, lane);
-        }
-        if (first || d2 != ent_dist(w_prev))
-            load_coax<T, P, W>(b2, coaxA + static_cast<int64_t>(d2) * NCF * 64, lane);
-    };
+    }
 
-    issue(0, 0, true);
-    for (int e = 0; e < ne; ++e) {
-        // ---- columns of entry e rotated by e^{i m phi} into X; operators of e
+    // Columns rotated by e^{i m phi} (cos, sin from cs) into X (degree-major);
+    // items this thread does not own, and the sine half of m = 0, go to the trash row.
+    __device__ __forceinline__ void store_cols(T* X, const Cols<GI>& c, const T* cs) const
+    {
+        constexpr int trash = K + kPadRows + kTrashRow;
 #pragma unroll
         for (int g = 0; g < GI; ++g) {
-            if (!gv[g]) continue;
             const int n = gnm[g] & 0xff, m = gnm[g] >> 8;
-            if (m > 0) {
-                Xs[ps_row(n, m, 0) * kLdsW + gc] = static_cast<T>(gx[g] * gcm[g] - gy[g] * gsm[g]);
-                Xs[ps_row(n, m, 1) * kLdsW + gc] = static_cast<T>(gx[g] * gsm[g] + gy[g] * gcm[g]);
-            } else {
-                Xs[ps_row(n, 0, 0) * kLdsW + gc] = static_cast<T>(gx[g]);
-            }
+            const double cm = cs[2 * m], sm = cs[2 * m + 1];
+            const double y = m > 0 ? c.y[g] : 0.0;
+            const int rc = gv[g] ? ps_row(n, m, 0) : trash;
+            const int rs = gv[g] && m > 0 ? ps_row(n, m, 1) : trash;
+            X[rc * kLdsW + gc] = static_cast<T>(c.x[g] * cm - y * sm);
+            X[rs * kLdsW + gc] = static_cast<T>(c.x[g] * sm + y * cm);
         }
-        if (tid < 2 * (P + 1)) cs_s[tid] = static_cast<T>(c3);
+    }
+
+    __device__ __forceinline__ void use_loaded()
+    {
 #pragma unroll
         for (int i = 0; i < (NK1 > 0 ? NK1 : 1); ++i) a1[i] = b1[i];
 #pragma unroll
         for (int i = 0; i < (NK2 > 0 ? NK2 : 1); ++i) a2[i] = b2[i];
 #pragma unroll
         for (int i = 0; i < (NK3 > 0 ? NK3 : 1); ++i) a3[i] = b3[i];
-        // ---- everything entry e+1 needs (the last entry reloads itself, harmlessly)
-        const int en = e + 1 < ne ? e + 1 : e;
-        issue(en, ent_w[e], false);
-        __syncthreads();
+    }
 
-        // ---- stage 1: polar rotation, degree-major X -> order-major Y
-        static_for<0, S.s1.count[W]>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            constexpr int item = S.s1.item[W][q];
-            constexpr int n = item >> 8, part = (item >> 4) & 15, mt = item & 15;
-            constexpr int nks = rot_nks(n, part);
-            constexpr int ko = s1_koff<P, W>(q);
-            constexpr int brow = part == 0 ? n * n : n * n + n + 1;
-            T bv[nks];
+    __device__ __forceinline__ void load_fwd(const M2LArgs& a, int cls, int lane)
+    {
+        load_rot_fwd<T, P, W>(b1, static_cast<const T*>(a.rotA) + (static_cast<int64_t>(cls) * 2 + 0) * NRF * 64, lane);
+    }
+    __device__ __forceinline__ void load_back(const M2LArgs& a, int cls, int lane)
+    {
+        load_rot_bwd<T, P, W>(b3, static_cast<const T*>(a.rotA) + (static_cast<int64_t>(cls) * 2 + 1) * NRF * 64, lane);
+    }
+    __device__ __forceinline__ void load_cx(const M2LArgs& a, int d2, int lane)
+    {
+        load_coax<T, P, W>(b2, static_cast<const T*>(a.coaxA) + static_cast<int64_t>(d2) * NCF * 64, lane);
+    }
+};
+
+// cos, sin of m * azimuth of the entry with word w, m = lane / 2 (lanes past
+// 2(p+1) read a valid element and are not stored).
+template <int P>
+__device__ __forceinline__ double load_cs(const M2LArgs& a, int w, int lane)
+{
+    return a.off_cs[ent_offset(w) * (P + 1) * 2 + (lane < 2 * (P + 1) ? lane : 0)];
+}
+
+// FP64 accumulators of a wave's stage-3 degrees: [item][m-tile][cos/sin][4]
+template <int NJ, int MT>
+struct Acc {
+    double v[NJ > 0 ? NJ : 1][MT][2][4];
+    __device__ __forceinline__ auto& operator[](int q) { return v[q]; }
+    __device__ __forceinline__ void zero()
+    {
 #pragma unroll
-            for (int ks = 0; ks < nks; ++ks) bv[ks] = Xs[(brow + 4 * ks + (lane >> 4)) * kLdsW + col];
-            V4 d = {0, 0, 0, 0};
+        for (int q = 0; q < (NJ > 0 ? NJ : 1); ++q)
 #pragma unroll
-            for (int ks = 0; ks < nks; ++ks) d = F::run(a1[ko + ks], bv[ks], d);
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[q][mt][0][r] = v[q][mt][1][r] = 0.0;
+    }
+};
+
+template <typename T, int P, int NC, int W, class A>
+__device__ __forceinline__ void write_locals(const M2LArgs& a, int64_t blk, double* __restrict__ L, A& acc, int lane)
+{
+    using F = Mfma<T>;
+    constexpr int K = (P + 1) * (P + 1);
+    constexpr int TB = 16 / NC;
+    constexpr auto S = schedule<P>();
+    const int col = lane & 15;
+    const int t = col / NC, c = col - (col / NC) * NC;
+    if (t >= TB) return;
+    const int32_t tg = a.blk_tgt[blk * TB + t];
+    if (tg < 0) return;
+    double* Lt = L + (static_cast<int64_t>(tg) * NC + c) * K;
+    static_for<0, S.s3.count[W]>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int j = S.s3.item[W][q] >> 8;
+#pragma unroll
+        for (int mt = 0; mt < rot_mts(j); ++mt)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * mt + F::drow(lane, r);
-                if (row <= n && (part == 0 || row > 0)) Ys[om_row(P, row, n, part) * kLdsW + col] = d[r];
-            }
-        });
-        __syncthreads();
-
-        // ---- stage 2: coaxial translation per order, order-major Y -> degree-major X
-        static_for<0, S.s2.count[W]>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            constexpr int item = S.s2.item[W][q];
-            constexpr int k = item >> 8, part = (item >> 4) & 15, mt = item & 15;
-            constexpr int nks = coax_nks(P, k);
-            constexpr int ko = s2_koff<P, W>(q);
-            constexpr int brow = om_row(P, k, k, part);
-            T bv[nks];
-#pragma unroll
-            for (int ks = 0; ks < nks; ++ks) bv[ks] = Ys[(brow + 4 * ks + (lane >> 4)) * kLdsW + col];
-            V4 d = {0, 0, 0, 0};
-#pragma unroll
-            for (int ks = 0; ks < nks; ++ks) d = F::run(a2[ko + ks], bv[ks], d);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int j = k + 16 * mt + F::drow(lane, r);
-                if (j <= P) Xs[ps_row(j, k, part) * kLdsW + col] = d[r];
-            }
-        });
-        __syncthreads();
-
-        // ---- stage 3: back rotation per degree, azimuth e^{-i m phi}, FP64 accumulation
-        static_for<0, NJ>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            constexpr int j = S.s3.item[W][q] >> 8;
-            constexpr int ko = s3_koff<P, W>(q);
-            constexpr int nkc = rot_nks(j, 0), nks = rot_nks(j, 1);
-            T bc[nkc], bs[nks > 0 ? nks : 1];
-#pragma unroll
-            for (int ks = 0; ks < nkc; ++ks) bc[ks] = Xs[(j * j + 4 * ks + (lane >> 4)) * kLdsW + col];
-#pragma unroll
-            for (int ks = 0; ks < nks; ++ks) bs[ks] = Xs[(j * j + j + 1 + 4 * ks + (lane >> 4)) * kLdsW + col];
-            static_for<0, rot_mts(j)>([&](auto mc) {
-                constexpr int mt = decltype(mc)::value;
-                constexpr int kb = ko + mt * (nkc + nks);
-                V4 dc = {0, 0, 0, 0}, ds = {0, 0, 0, 0};
-#pragma unroll
-                for (int ks = 0; ks < nkc; ++ks) dc = F::run(a3[kb + ks], bc[ks], dc);
-#pragma unroll
-                for (int ks = 0; ks < nks; ++ks) ds = F::run(a3[kb + nkc + ks], bs[ks], ds);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = 16 * mt + F::drow(lane, r);
-                    if (row <= j) {
-                        const double cm = cs_s[2 * row], sm = cs_s[2 * row + 1];
-                        const double lc = dc[r], ls = ds[r];
-                        acc[q][mt][0][r] += lc * cm + ls * sm;
-                        acc[q][mt][1][r] += ls * cm - lc * sm;
-                    }
+                if (row <= j) {
+                    Lt[rfc2(j, row)] = acc[q][mt][0][r];
+                    if (row > 0) Lt[rfs2(j, row)] = acc[q][mt][1][r];
                 }
-            });
-        });
-        __syncthreads();
-    }
+            }
+    });
+}
 
-    const int t = col / NC, c = col - (col / NC) * NC;
-    if (t < TB) {
-        const int32_t tg = a.blk_tgt[blk * TB + t];
-        if (tg >= 0) {
-            double* Lt = L + (static_cast<int64_t>(tg) * NC + c) * K;
-            static_for<0, NJ>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                constexpr int j = S.s3.item[W][q] >> 8;
-#pragma unroll
-                for (int mt = 0; mt < rot_mts(j); ++mt)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = 16 * mt + F::drow(lane, r);
-                        if (row <= j) {
-                            Lt[rfc2(j, row)] = acc[q][mt][0][r];
-                            if (row > 0) Lt[rfs2(j, row)] = acc[q][mt][1][r];
-                        }
-                    }
-            });
-        }
+// ---------------------------------------------------------------- pipelined kernel
+// Entries flow through a three-deep software pipeline: in phase i a wave
+// stores the columns of entry i, runs stage 1 on entry i-1, stage 2 on entry
+// i-2 and stage 3 on entry i-3; one barrier ends the phase.  Stage buffers
+// alternate by entry parity (G: stored columns, Y: after stage 1, Z: after
+// stage 2), so no two concurrent steps touch the same buffer.  Loads run ahead:
+// source columns kAhead<T> entries (with two, two register sets alternate by
+// phase parity), operators and the azimuth table one entry.  Entries outside [0, ne) run on
+// zeroed or repeated data with zero azimuth factors, so they add nothing.
+constexpr int kCsSlots = 8;  // azimuth table slots in LDS, by entry index mod 8
+
+// Column prefetch distance: two entries where the registers allow it (FP32
+// stages), one for FP64 (a second set would spill at p = 12).
+template <typename T>
+constexpr int kAhead = sizeof(T) == 4 ? 2 : 1;
+
+template <typename T, int P, int NC>
+__host__ __device__ constexpr size_t pipe_lds_bytes()
+{
+    return 6 * static_cast<size_t>((P + 1) * (P + 1) + kPadRows + 1) * kLdsW * sizeof(T) +
+           (kCsSlots * 2 * (P + 1) + 64) * sizeof(T) + static_cast<size_t>(kMaxEntries) * (1 + 16 / NC) * sizeof(int);
+}
+
+template <typename T, int P, int NC>
+__host__ __device__ constexpr bool pipe_fits()
+{
+    return pipe_lds_bytes<T, P, NC>() <= 160 * 1024;
+}
+
+template <typename T, int P, int NC, int W>
+__device__ __forceinline__ void m2l_pipe_wave(const M2LArgs& a, int64_t blk, int ne, const double* __restrict__ M,
+                                              double* __restrict__ L, T* buf, T* cst, const int* ent_w,
+                                              const int* ent_s)
+{
+    using WS = WaveState<T, P, NC, W>;
+    constexpr int K = (P + 1) * (P + 1);
+    constexpr int BS = (K + kPadRows + 1) * kLdsW;
+    constexpr int TB = 16 / NC;
+    constexpr int NCS = 2 * (P + 1);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    T* G = buf;
+    T* Y = buf + 2 * BS;
+    T* Z = buf + 4 * BS;
+    auto ce = [&](int k) { return k < 0 ? 0 : (k >= ne ? ne - 1 : k); };
+    auto cs_slot = [&](int k) { return cst + (k & (kCsSlots - 1)) * NCS; };
+
+    Acc<WS::NJ, rot_mts(P)> acc;
+    acc.zero();
+    WS ws;
+    ws.init(tid);
+    constexpr int AH = kAhead<T>;
+    Cols<WS::GI> c0, c1;  // columns of the entries ahead (c1 unused with one)
+    ws.load_cols(c0, a, M, ent_s + ce(0) * TB);
+    if constexpr (AH == 2) ws.load_cols(c1, a, M, ent_s + ce(1) * TB);
+    double cs_next = 0.0;
+    if constexpr (W == 0) {
+        const double v = load_cs<P>(a, ent_w[0], lane);
+        cs_next = load_cs<P>(a, ent_w[ce(1)], lane);
+        if (lane < NCS) cs_slot(0)[lane] = static_cast<T>(v);
     }
+    {
+        const int w = ent_w[0];
+        const int cls = __builtin_amdgcn_readfirstlane(ent_class(w));
+        const int d2 = __builtin_amdgcn_readfirstlane(ent_dist(w));
+        ws.load_fwd(a, cls, lane);
+        ws.load_cx(a, d2, lane);
+        ws.load_back(a, cls, lane);
+    }
+    __syncthreads();
+
+    auto phase = [&](int i, Cols<WS::GI>& cols) {
+        // ---- columns of entry i (loaded two phases ago); azimuth table of entry i+1
+        ws.store_cols(G + (i & 1) * BS, cols, cs_slot(i));
+        if constexpr (W == 0) {  // lanes past the table write the trash tail
+            T* dst = lane < NCS ? cs_slot(i + 1) + lane : cst + kCsSlots * NCS + lane;
+            *dst = i + 1 < ne ? static_cast<T>(cs_next) : T(0);
+            cs_next = load_cs<P>(a, ent_w[ce(i + 2)], lane);
+        }
+        ws.use_loaded();
+        // ---- loads: columns of entry i+AH, operators of entries i, i-1, i-2
+        ws.load_cols(cols, a, M, ent_s + ce(i + AH) * TB);
+        const int w0 = ent_w[ce(i)], w1 = ent_w[ce(i - 1)], w2 = ent_w[ce(i - 2)], w3 = ent_w[ce(i - 3)];
+        const int k0 = __builtin_amdgcn_readfirstlane(ent_class(w0));
+        const int k1 = __builtin_amdgcn_readfirstlane(ent_class(w1));
+        const int k2 = __builtin_amdgcn_readfirstlane(ent_class(w2));
+        const int k3 = __builtin_amdgcn_readfirstlane(ent_class(w3));
+        const int d1 = __builtin_amdgcn_readfirstlane(ent_dist(w1));
+        const int d2 = __builtin_amdgcn_readfirstlane(ent_dist(w2));
+        if (k0 != k1) ws.load_fwd(a, k0, lane);
+        if (d1 != d2) ws.load_cx(a, d1, lane);
+        if (k2 != k3) ws.load_back(a, k2, lane);
+        // ---- the three stages on entries i-1, i-2, i-3
+        StageIn<T, P, W> in;
+        read_rot_fwd<T, P, W>(in.b1, G + ((i - 1) & 1) * BS, lane);
+        read_coax<T, P, W>(in.b2, Y + (i & 1) * BS, lane);
+        read_rot_back<T, P, W>(in.b3, Z + ((i - 1) & 1) * BS, lane);
+        stage_rot_back<T, P, W>(ws.a3, in.b3, cs_slot(i - 3), acc, lane);
+        stage_rot_fwd<T, P, W>(ws.a1, in.b1, Y + ((i - 1) & 1) * BS, lane);
+        stage_coax<T, P, W>(ws.a2, in.b2, Z + (i & 1) * BS, lane);
+        __syncthreads();
+    };
+    // ne + 3 phases drain the pipeline; with two column sets one more phase
+    // (even count) adds nothing
+    if constexpr (AH == 2) {
+        for (int i = 0; i < ne + 3; i += 2) {
+            phase(i, c0);
+            phase(i + 1, c1);
+        }
+    } else {
+        for (int i = 0; i < ne + 3; ++i) phase(i, c0);
+    }
+    write_locals<T, P, NC, W>(a, blk, L, acc, lane);
 }
 
 // Targets with an empty interaction list get L = 0.
@@ -387,21 +678,136 @@ __device__ __forceinline__ void m2l_zero(const M2LArgs& a, int64_t blk, double* 
     }
 }
 
+// Block index in XCD-aware order: consecutive blocks (children of nearby
+// parents, all octants) run on the same XCD and share its L2 for sources and
+// operators.
+__device__ __forceinline__ int64_t xcd_block(int64_t n_block)
+{
+    const int64_t per = (n_block + 7) / 8;
+    return static_cast<int64_t>(blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
 template <typename T, int P, int NC>
-__global__ __launch_bounds__(kThreads, 1) void m2l_mfma_kernel(M2LArgs a, const double* __restrict__ M,
+__global__ __launch_bounds__(kThreads, 1) void m2l_pipe_kernel(M2LArgs a, const double* __restrict__ M,
                                                               double* __restrict__ L)
 {
     constexpr int K = (P + 1) * (P + 1);
     constexpr int TB = 16 / NC;
-    __shared__ T Xs[(K + kPadRows) * kLdsW];
-    __shared__ T Ys[(K + kPadRows) * kLdsW];
-    __shared__ T cs_s[2 * (P + 1)];
+    constexpr int BS = (K + kPadRows + 1) * kLdsW;
+    constexpr int NCST = kCsSlots * 2 * (P + 1) + 64;
+    __shared__ T buf[pipe_fits<T, P, NC>() ? 6 * BS : 1];
+    __shared__ T cst[NCST];
     __shared__ int ent_w[kMaxEntries];
     __shared__ int ent_s[kMaxEntries * TB];
-    // XCD-aware order: consecutive blocks (children of nearby parents, all
-    // octants) run on the same XCD and share its L2 for sources and operators.
-    const int64_t per = (a.n_block + 7) / 8;
-    const int64_t blk = static_cast<int64_t>(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if constexpr (pipe_fits<T, P, NC>()) {
+        const int64_t blk = xcd_block(a.n_block);
+        if (blk >= a.n_block) return;
+        const int64_t e0 = a.blk_row[blk];
+        const int ne = static_cast<int>(a.blk_row[blk + 1] - e0);  // <= kMaxEntries (host-checked)
+        if (ne == 0) {
+            m2l_zero<P, NC>(a, blk, L);
+            return;
+        }
+        for (int i = threadIdx.x; i < 6 * BS; i += kThreads) buf[i] = T(0);
+        for (int i = threadIdx.x; i < NCST; i += kThreads) cst[i] = T(0);
+        for (int i = threadIdx.x; i < ne; i += kThreads) ent_w[i] = a.ent_meta[e0 + i];
+        for (int i = threadIdx.x; i < ne * TB; i += kThreads) ent_s[i] = a.ent_src[e0 * TB + i];
+        __syncthreads();
+        switch (threadIdx.x >> 6) {
+        case 0: m2l_pipe_wave<T, P, NC, 0>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
+        case 1: m2l_pipe_wave<T, P, NC, 1>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
+        case 2: m2l_pipe_wave<T, P, NC, 2>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
+        case 3: m2l_pipe_wave<T, P, NC, 3>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
+        case 4: m2l_pipe_wave<T, P, NC, 4>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
+        case 5: m2l_pipe_wave<T, P, NC, 5>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
+        case 6: m2l_pipe_wave<T, P, NC, 6>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
+        default: m2l_pipe_wave<T, P, NC, 7>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- sequential kernel
+// For orders whose six pipeline buffers exceed LDS (FP64, p > 12): one entry
+// at a time, four barriers per entry, loads issued one entry ahead.
+template <typename T, int P, int NC, int W>
+__device__ __forceinline__ void m2l_seq_wave(const M2LArgs& a, int64_t blk, int ne, const double* __restrict__ M,
+                                             double* __restrict__ L, T* Xs, T* Ys, T* cst, const int* ent_w,
+                                             const int* ent_s)
+{
+    using WS = WaveState<T, P, NC, W>;
+    constexpr int TB = 16 / NC;
+    constexpr int NCS = 2 * (P + 1);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    Acc<WS::NJ, rot_mts(P)> acc;
+    acc.zero();
+    WS ws;
+    ws.init(tid);
+    Cols<WS::GI> cols;
+    ws.load_cols(cols, a, M, ent_s);
+    double cs_next = 0.0;
+    if constexpr (W == 0) {
+        const double v = load_cs<P>(a, ent_w[0], lane);
+        cs_next = load_cs<P>(a, ent_w[ne > 1 ? 1 : 0], lane);
+        if (lane < NCS) cst[lane] = static_cast<T>(v);
+    }
+    {
+        const int w = ent_w[0];
+        const int cls = __builtin_amdgcn_readfirstlane(ent_class(w));
+        const int d2 = __builtin_amdgcn_readfirstlane(ent_dist(w));
+        ws.load_fwd(a, cls, lane);
+        ws.load_cx(a, d2, lane);
+        ws.load_back(a, cls, lane);
+    }
+    __syncthreads();
+    for (int e = 0; e < ne; ++e) {
+        // azimuth table: slot e & 1 holds entry e, slot (e + 1) & 1 receives entry e + 1
+        const T* cs = cst + (e & 1) * NCS;
+        ws.store_cols(Xs, cols, cs);
+        if constexpr (W == 0) {
+            cst[lane < NCS ? ((e + 1) & 1) * NCS + lane : 2 * NCS + lane] = static_cast<T>(cs_next);
+            cs_next = load_cs<P>(a, ent_w[e + 2 < ne ? e + 2 : ne - 1], lane);
+        }
+        ws.use_loaded();
+        const int en = e + 1 < ne ? e + 1 : e;  // the last entry reloads itself, harmlessly
+        ws.load_cols(cols, a, M, ent_s + en * TB);
+        const int wn = ent_w[en], we = ent_w[e];
+        const int cn = __builtin_amdgcn_readfirstlane(ent_class(wn));
+        const int dn = __builtin_amdgcn_readfirstlane(ent_dist(wn));
+        const int c_e = __builtin_amdgcn_readfirstlane(ent_class(we));
+        const int d_e = __builtin_amdgcn_readfirstlane(ent_dist(we));
+        if (cn != c_e) {
+            ws.load_fwd(a, cn, lane);
+            ws.load_back(a, cn, lane);
+        }
+        if (dn != d_e) ws.load_cx(a, dn, lane);
+        __syncthreads();
+        StageIn<T, P, W> in;
+        read_rot_fwd<T, P, W>(in.b1, Xs, lane);
+        stage_rot_fwd<T, P, W>(ws.a1, in.b1, Ys, lane);
+        __syncthreads();
+        read_coax<T, P, W>(in.b2, Ys, lane);
+        stage_coax<T, P, W>(ws.a2, in.b2, Xs, lane);
+        __syncthreads();
+        read_rot_back<T, P, W>(in.b3, Xs, lane);
+        stage_rot_back<T, P, W>(ws.a3, in.b3, cs, acc, lane);
+        __syncthreads();
+    }
+    write_locals<T, P, NC, W>(a, blk, L, acc, lane);
+}
+
+template <typename T, int P, int NC>
+__global__ __launch_bounds__(kThreads, 1) void m2l_seq_kernel(M2LArgs a, const double* __restrict__ M,
+                                                             double* __restrict__ L)
+{
+    constexpr int K = (P + 1) * (P + 1);
+    constexpr int TB = 16 / NC;
+    __shared__ T Xs[(K + kPadRows + 1) * kLdsW];
+    __shared__ T Ys[(K + kPadRows + 1) * kLdsW];
+    __shared__ T cst[4 * (P + 1) + 64];
+    __shared__ int ent_w[kMaxEntries];
+    __shared__ int ent_s[kMaxEntries * TB];
+    const int64_t blk = xcd_block(a.n_block);
     if (blk >= a.n_block) return;
     const int64_t e0 = a.blk_row[blk];
     const int ne = static_cast<int>(a.blk_row[blk + 1] - e0);  // <= kMaxEntries (host-checked)
@@ -417,26 +823,35 @@ __global__ __launch_bounds__(kThreads, 1) void m2l_mfma_kernel(M2LArgs a, const 
     for (int i = threadIdx.x; i < ne * TB; i += kThreads) ent_s[i] = a.ent_src[e0 * TB + i];
     __syncthreads();
     switch (threadIdx.x >> 6) {
-    case 0: m2l_wave<T, P, NC, 0>(a, blk, ne, M, L, Xs, Ys, cs_s, ent_w, ent_s); break;
-    case 1: m2l_wave<T, P, NC, 1>(a, blk, ne, M, L, Xs, Ys, cs_s, ent_w, ent_s); break;
-    case 2: m2l_wave<T, P, NC, 2>(a, blk, ne, M, L, Xs, Ys, cs_s, ent_w, ent_s); break;
-    case 3: m2l_wave<T, P, NC, 3>(a, blk, ne, M, L, Xs, Ys, cs_s, ent_w, ent_s); break;
-    case 4: m2l_wave<T, P, NC, 4>(a, blk, ne, M, L, Xs, Ys, cs_s, ent_w, ent_s); break;
-    case 5: m2l_wave<T, P, NC, 5>(a, blk, ne, M, L, Xs, Ys, cs_s, ent_w, ent_s); break;
-    case 6: m2l_wave<T, P, NC, 6>(a, blk, ne, M, L, Xs, Ys, cs_s, ent_w, ent_s); break;
-    default: m2l_wave<T, P, NC, 7>(a, blk, ne, M, L, Xs, Ys, cs_s, ent_w, ent_s); break;
+    case 0: m2l_seq_wave<T, P, NC, 0>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
+    case 1: m2l_seq_wave<T, P, NC, 1>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
+    case 2: m2l_seq_wave<T, P, NC, 2>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
+    case 3: m2l_seq_wave<T, P, NC, 3>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
+    case 4: m2l_seq_wave<T, P, NC, 4>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
+    case 5: m2l_seq_wave<T, P, NC, 5>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
+    case 6: m2l_seq_wave<T, P, NC, 6>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
+    default: m2l_seq_wave<T, P, NC, 7>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
     }
+}
+
+template <typename T, int P, int NC>
+void launch_one(const M2LArgs& a, const double* M, double* L, hipStream_t s)
+{
+    const unsigned grid = static_cast<unsigned>(((a.n_block + 7) / 8) * 8);
+    if constexpr (pipe_fits<T, P, NC>())
+        hipLaunchKernelGGL((m2l_pipe_kernel<T, P, NC>), dim3(grid), dim3(kThreads), 0, s, a, M, L);
+    else
+        hipLaunchKernelGGL((m2l_seq_kernel<T, P, NC>), dim3(grid), dim3(kThreads), 0, s, a, M, L);
 }
 
 template <typename T, int P>
 void launch_nc(const M2LArgs& a, int nc, const double* M, double* L, hipStream_t s)
 {
-    const unsigned grid = static_cast<unsigned>(((a.n_block + 7) / 8) * 8);
     switch (nc) {
-    case 1: hipLaunchKernelGGL((m2l_mfma_kernel<T, P, 1>), dim3(grid), dim3(kThreads), 0, s, a, M, L); break;
-    case 2: hipLaunchKernelGGL((m2l_mfma_kernel<T, P, 2>), dim3(grid), dim3(kThreads), 0, s, a, M, L); break;
-    case 3: hipLaunchKernelGGL((m2l_mfma_kernel<T, P, 3>), dim3(grid), dim3(kThreads), 0, s, a, M, L); break;
-    default: hipLaunchKernelGGL((m2l_mfma_kernel<T, P, 4>), dim3(grid), dim3(kThreads), 0, s, a, M, L); break;
+    case 1: launch_one<T, P, 1>(a, M, L, s); break;
+    case 2: launch_one<T, P, 2>(a, M, L, s); break;
+    case 3: launch_one<T, P, 3>(a, M, L, s); break;
+    default: launch_one<T, P, 4>(a, M, L, s); break;
     }
 }
 
