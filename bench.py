@@ -101,6 +101,8 @@ def main() -> int:
     ap.add_argument("--targets", type=int, default=10_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="first run a device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is not built)",
@@ -121,6 +123,15 @@ def main() -> int:
     if not torch.cuda.is_available():
         log("[bench] no GPU visible")
         return 3
+    if args.copy_bytes:
+        # a plain copy of known size: the HBM counters of its kernel calibrate
+        # FETCH_SIZE / WRITE_SIZE for the kernels below
+        src = torch.ones(args.copy_bytes // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {args.copy_bytes} bytes")
+        del src, dst
     pos, q = getattr(W, gen)(n, seed=seed)
     t0 = time.time()
     part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, cap)

@@ -41,6 +41,11 @@ enum {
 
 /* Which part of the sum (rinv = both, rinv_nonear = far, N2N = near). */
 enum { SFMM_FAR = 1, SFMM_NEAR = 2, SFMM_BOTH = 3 };
+/* or-ed into `which`: the near field excludes the pairs of the plan's self lists
+ * (rinv(Q, U, is_self) with is_self(i, j) = j in self_generator(i), fmm.hpp:864-873);
+ * without it only j == i is excluded (rinv(Q, U), fmm.hpp:875-881).  Listed pairs in
+ * the far field are subtracted either way (_slist, fmm.hpp:398-408,798-807). */
+enum { SFMM_SELF_LIST = 4 };
 
 enum { SFMM_FP64 = 0, SFMM_MIXED = 1 };
 
@@ -88,9 +93,9 @@ typedef struct {
     int p;                    /* expansion order, 1..16 (FMM3D template p)            */
     int precision;            /* SFMM_FP64 or SFMM_MIXED (FP32 MFMA M2L stages, FP64 accumulation) */
     int max_ncharge;          /* largest ncharge later passed to SFMM_POTENTIAL (1..4) */
-    /* optional self exclusions (FMM3D(partitioner, comm, self_generator), fmm.hpp:204):
-     * for particle i, pairs (i, j), j in self_val[self_row[i] .. self_row[i+1]), are
-     * excluded; NULL means j == i only. */
+    /* optional self lists (FMM3D(partitioner, comm, self_generator), fmm.hpp:204):
+     * self_generator(i) = self_val[self_row[i] .. self_row[i+1]) (original indices,
+     * self_row[0] = 0); NULL means self_generator(i) = {i}.  Copied at creation. */
     const int64_t* self_row;
     const int64_t* self_val;
     int m2l_kernel;           /* 0: offset-grouped MFMA (default); 1: row-parallel VALU kernel */

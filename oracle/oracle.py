@@ -46,6 +46,8 @@ def lib():
         L.oracle_fmm_self.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int,
                                       ctypes.c_int, _ip, _ip]
         L.oracle_direct.argtypes = [ctypes.c_int64, _dp, ctypes.c_int, _dp, ctypes.c_int64, _ip, _dp]
+        L.oracle_is_neighbor.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]
+        L.oracle_is_neighbor.restype = ctypes.c_int
         L.oracle_Z.restype = ctypes.c_double
         L.oracle_Z.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.oracle_A.restype = ctypes.c_double
@@ -105,6 +107,10 @@ class Tree:
                 "plist", "nlist_row", "nlist_val"]
         lib().oracle_tree_copy(self.h, l, *[_i(out[k]) for k in keys])
         return out
+
+    def is_neighbor(self, i: int, j: int) -> bool:
+        """OctreePartitioner::is_neighbor(i, j) (partition.hpp:484-488)."""
+        return bool(lib().oracle_is_neighbor(self.h, int(i), int(j)))
 
     def fmm(self, p: int, q, mode=POTGRAD, which=BOTH, dense=False, self_lists=None):
         q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)

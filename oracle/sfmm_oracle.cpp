@@ -838,6 +838,11 @@ void oracle_tree_free(void* tree) { delete static_cast<Tree*>(tree); }
 
 int oracle_tree_depth(const void* tree) { return static_cast<const Tree*>(tree)->depth; }
 
+int oracle_is_neighbor(const void* tree, int64_t i, int64_t j)
+{
+    return nlist_contains(*static_cast<const Tree*>(tree), static_cast<int>(i), static_cast<int>(j)) ? 1 : 0;
+}
+
 void oracle_tree_box(const void* tree, double* out4)
 {
     const Tree* t = static_cast<const Tree*>(tree);

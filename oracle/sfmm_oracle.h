@@ -32,6 +32,8 @@ enum { ORACLE_FAR = 1, ORACLE_NEAR = 2, ORACLE_BOTH = 3 };
 void* oracle_tree_build(int64_t n, const double* pos, int max_level, int cap, int vertical_axis, double phideg);
 void oracle_tree_free(void* tree);
 int oracle_tree_depth(const void* tree);
+/* OctreePartitioner::is_neighbor(i, j) (partition.hpp:484-488): j's leaf is in the near list of i's leaf. */
+int oracle_is_neighbor(const void* tree, int64_t i, int64_t j);
 /* box: width, corner x, y, z */
 void oracle_tree_box(const void* tree, double* out4);
 /* counts: n_node, n_leaf, nnz(indices), nnz(ilist), nnz(clist), nnz(nlist) */

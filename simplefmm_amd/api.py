@@ -27,6 +27,7 @@ LIB_PATH = os.path.join(_HERE, "libsfmm.so")
 
 POTENTIAL, CURL, POTGRAD = 0, 1, 2
 FAR, NEAR, BOTH = 1, 2, 3
+SELF_LIST = 4  # or-ed into `which`: the near field excludes the plan's self lists
 FP64, MIXED = 0, 1
 STAGES = ("gather", "P2M", "M2M", "M2L", "L2L", "L2P", "P2P", "scatter", "total")
 
@@ -207,7 +208,9 @@ class FMM3D:
     """
 
     def __init__(self, partitioner: OctreePartitioner, p: int, ncharge: int = 3, precision: int = FP64,
-                 m2l_kernel: int = 0):
+                 m2l_kernel: int = 0, self_lists=None):
+        """self_lists: optional (row, val) CSR of self_generator(i) (fmm.hpp:204),
+        original particle indices; None means self_generator(i) = {i}."""
         if partitioner.h is None:
             raise SfmmError("refine() the partitioner before building an FMM3D")
         self.partitioner = partitioner  # the plan references the tree: keep it alive
@@ -218,6 +221,13 @@ class FMM3D:
         desc.precision = int(precision)
         desc.max_ncharge = max(1, min(4, self.ncharge))
         desc.m2l_kernel = int(m2l_kernel)
+        if self_lists is not None:
+            row = np.ascontiguousarray(self_lists[0], dtype=np.int64)
+            val = np.ascontiguousarray(self_lists[1], dtype=np.int64)
+            if row.shape != (partitioner.n + 1,):
+                raise ValueError("self_lists row must have n + 1 entries")
+            desc.self_row = _i(row)
+            desc.self_val = _i(val)
         h = ctypes.c_void_p()
         _check(lib().sfmm_plan_create(partitioner.h, ctypes.byref(desc), ctypes.byref(h)))
         self.h = h

@@ -8,8 +8,11 @@
 //   * the communicator argument is accepted and ignored (one process, one GPU);
 //   * T must be double; p must be in 1..16; gradient (curl) needs N == 3, as in
 //     the reference (fmm.hpp:550);
-//   * rinv(Q, U, is_self) honours the exclusions given by the self_generator at
-//     construction (the default is i == j); is_self itself is not evaluated;
+//   * rinv(Q, U, is_self) and N2N(Q, U, is_self) take is_self(i, j) to be
+//     "j is in self_generator(i)" (the generator given at construction; by
+//     default i -> {i}); is_self itself is not evaluated.  rinv(Q, U) excludes
+//     only j == i in the near field, as in the reference; far-field pairs of
+//     the generator are subtracted in both (_slist);
 //   * a failing call prints the library's message and aborts, where the
 //     reference would assert.
 #pragma once
@@ -214,7 +217,7 @@ public:
     template <bool gradient = false, typename IsSelf>
     void rinv(const Container<T, N>* Q, Container<T, N>* U, const IsSelf&) const noexcept
     {
-        eval<gradient>(Q, U, SFMM_BOTH);
+        eval<gradient>(Q, U, SFMM_BOTH | SFMM_SELF_LIST);
     }
 
     template <bool gradient = false>
@@ -226,7 +229,7 @@ public:
     template <bool gradient = false, typename IsSelf>
     void N2N(const Container<T, N>* Q, Container<T, N>* U, const IsSelf&) const noexcept
     {
-        eval<gradient>(Q, U, SFMM_NEAR);
+        eval<gradient>(Q, U, SFMM_NEAR | SFMM_SELF_LIST);
     }
 
     // Scalar charges q[i]: out[4i..4i+3] += phi, dphi/dx, dphi/dy, dphi/dz.

@@ -72,6 +72,11 @@ struct sfmm_plan_s {
     const float* rotA32 = nullptr;
     const float* coaxA32 = nullptr;
     const double* zero_cols = nullptr;  // 4 (p+1)^2 zeros: M2L columns of absent sources
+    const double* m2m_frag = nullptr;   // matrix-core M2M / L2L operators [8][nfrag][64]
+    const double* l2l_frag = nullptr;
+    sfmm::TransArgs m2m[5], l2l[5];     // per component count (built on first use)
+    sfmm::SelfCorr self;                // self-list corrections (empty without a self generator)
+    std::vector<int64_t> m2m_lvl[5], l2l_lvl[5];  // first block of each level (+ end)
     hipEvent_t* last_ev = nullptr;   // event set of the last evaluation
     std::vector<hipEvent_t> rec_ev;  // sets of 9 events, one set per recorded evaluation
     int rec_count = 0, rec_used = 0;
@@ -239,18 +244,109 @@ int build_m2l_blocks(sfmm_plan_s* P, int nc)
     return SFMM_OK;
 }
 
+// M2M blocks: parents of one level with children, 16/nc per block, one entry
+// per child octant present.  L2L blocks: nodes of one level grouped by octant,
+// 16/nc per block, one entry (the octant; inputs are the parents).
+int build_translate_blocks(sfmm_plan_s* P, int nc)
+{
+    if (P->m2m[nc].blk_row) return SFMM_OK;
+    const sfmm::Tree& t = P->tree->t;
+    const int TB = 16 / nc;
+    std::vector<int64_t> row{0}, lrow{0};
+    std::vector<int32_t> out, oct, in, lout, loct, lin;
+    auto& lv_m = P->m2m_lvl[nc];
+    auto& lv_l = P->l2l_lvl[nc];
+    lv_m.assign(t.depth + 2, 0);
+    lv_l.assign(t.depth + 2, 0);
+    for (int l = 0; l <= t.depth; ++l) {
+        const auto& lv = t.lv[l];
+        lv_m[l] = static_cast<int64_t>(row.size()) - 1;
+        if (l < t.depth) {
+            std::vector<int32_t> parents;
+            for (int32_t a = 0; a < lv.n_node(); ++a)
+                if (lv.nchild[a] > 0) parents.push_back(a);
+            for (size_t b = 0; b < parents.size(); b += TB) {
+                int32_t child[8][16];
+                for (auto& r : child) std::fill(r, r + 16, -1);
+                bool present[8] = {};
+                for (int tt = 0; tt < TB; ++tt) {
+                    if (b + tt >= parents.size()) {
+                        out.push_back(-1);
+                        continue;
+                    }
+                    const int32_t a = parents[b + tt];
+                    out.push_back(static_cast<int32_t>(P->level_first[l] + a));
+                    const auto& cl = t.lv[l + 1];
+                    for (int c = 0; c < lv.nchild[a]; ++c) {
+                        const int32_t ci = lv.child0[a] + c;
+                        const int o = static_cast<int>(cl.code[ci] & 7);
+                        child[o][tt] = static_cast<int32_t>(P->level_first[l + 1] + ci);
+                        present[o] = true;
+                    }
+                }
+                for (int o = 0; o < 8; ++o) {
+                    if (!present[o]) continue;
+                    oct.push_back(o);
+                    in.insert(in.end(), child[o], child[o] + TB);
+                }
+                row.push_back(static_cast<int64_t>(oct.size()));
+            }
+        }
+        lv_l[l] = static_cast<int64_t>(lrow.size()) - 1;
+        if (l >= 1) {
+            std::vector<int32_t> by_oct[8];
+            for (int32_t a = 0; a < lv.n_node(); ++a) by_oct[lv.code[a] & 7].push_back(a);
+            for (int o = 0; o < 8; ++o)
+                for (size_t b = 0; b < by_oct[o].size(); b += TB) {
+                    loct.push_back(o);
+                    for (int tt = 0; tt < TB; ++tt) {
+                        if (b + tt >= by_oct[o].size()) {
+                            lout.push_back(-1);
+                            lin.push_back(-1);
+                            continue;
+                        }
+                        const int32_t a = by_oct[o][b + tt];
+                        lout.push_back(static_cast<int32_t>(P->level_first[l] + a));
+                        lin.push_back(static_cast<int32_t>(P->level_first[l - 1] + lv.parent[a]));
+                    }
+                    lrow.push_back(static_cast<int64_t>(loct.size()));
+                }
+        }
+    }
+    lv_m[t.depth + 1] = static_cast<int64_t>(row.size()) - 1;
+    lv_l[t.depth + 1] = static_cast<int64_t>(lrow.size()) - 1;
+    sfmm::TransArgs& A = P->m2m[nc];
+    sfmm::TransArgs& B = P->l2l[nc];
+    int rc;
+    if ((rc = P->upload(row, &A.blk_row)) || (rc = P->upload(out, &A.blk_out)) || (rc = P->upload(oct, &A.ent_oct)) ||
+        (rc = P->upload(in, &A.ent_in)) || (rc = P->upload(lrow, &B.blk_row)) || (rc = P->upload(lout, &B.blk_out)) ||
+        (rc = P->upload(loct, &B.ent_oct)) || (rc = P->upload(lin, &B.ent_in)))
+        return rc;
+    A.frag = P->m2m_frag;
+    B.frag = P->l2l_frag;
+    return SFMM_OK;
+}
+
+sfmm::LevelSpan level_blocks(const std::vector<int64_t>& first, int l)
+{
+    return {first[l], first[l + 1] - first[l]};
+}
+
 int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int which)
 {
     sfmm::Mode mode;
     int qdim, nc;
     if (int rc = mode_of(kind, ncharge, mode, qdim, nc)) return rc;
-    if (which < 1 || which > 3) return fail(SFMM_ERR_ARG, "which must be SFMM_FAR, SFMM_NEAR or SFMM_BOTH");
+    if ((which & ~(SFMM_BOTH | SFMM_SELF_LIST)) != 0 || (which & SFMM_BOTH) == 0)
+        return fail(SFMM_ERR_ARG, "which must be SFMM_FAR, SFMM_NEAR or SFMM_BOTH, optionally | SFMM_SELF_LIST");
     if (mode == sfmm::MODE_POT && ncharge > P->max_ncharge)
         return fail(SFMM_ERR_ARG, "ncharge exceeds the plan's max_ncharge");
     HIP_TRY(hipSetDevice(P->device));
     if (int rc = ensure_buffers(P, nc, qdim)) return rc;
-    if (P->m2l_kernel == 0)
+    if (P->m2l_kernel == 0) {
         if (int rc = build_m2l_blocks(P, nc)) return rc;
+        if (int rc = build_translate_blocks(P, nc)) return rc;
+    }
     const sfmm::DevPlan& d = P->d;
     hipStream_t s = P->stream;
     const int depth = P->tree->t.depth;
@@ -264,8 +360,12 @@ int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int
     if (far) sfmm::launch_p2m(d, mode, nc, P->qbuf, P->M, s);
     HIP_TRY(hipEventRecord(ev[2], s));
     if (far)
-        for (int l = depth - 1; l >= 2; --l)
-            sfmm::launch_m2m(d, {P->level_first[l], P->level_first[l + 1] - P->level_first[l]}, nc, P->M, s);
+        for (int l = depth - 1; l >= 2; --l) {
+            if (P->m2l_kernel == 0)
+                sfmm::launch_translate(P->m2m[nc], level_blocks(P->m2m_lvl[nc], l), P->p, nc, false, P->M, s);
+            else
+                sfmm::launch_m2m(d, {P->level_first[l], P->level_first[l + 1] - P->level_first[l]}, nc, P->M, s);
+        }
     HIP_TRY(hipEventRecord(ev[3], s));
     if (far) {
         if (P->m2l_kernel == 0) sfmm::launch_m2l_mfma(P->m2l[nc], P->p, nc, P->M, P->L, s);
@@ -273,13 +373,19 @@ int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int
     }
     HIP_TRY(hipEventRecord(ev[4], s));
     if (far)
-        for (int l = 3; l <= depth; ++l)
-            sfmm::launch_l2l(d, {P->level_first[l], P->level_first[l + 1] - P->level_first[l]}, nc, P->L, s);
+        for (int l = 3; l <= depth; ++l) {
+            if (P->m2l_kernel == 0)
+                sfmm::launch_translate(P->l2l[nc], level_blocks(P->l2l_lvl[nc], l), P->p, nc, true, P->L, s);
+            else
+                sfmm::launch_l2l(d, {P->level_first[l], P->level_first[l + 1] - P->level_first[l]}, nc, P->L, s);
+        }
     HIP_TRY(hipEventRecord(ev[5], s));
     if (far) sfmm::launch_l2p(d, nc, P->L, P->obuf, s);
     HIP_TRY(hipEventRecord(ev[6], s));
-    if (which & SFMM_NEAR)
+    if (which & SFMM_NEAR) {
         sfmm::launch_p2p(d, P->p2p, mode, nc, P->qbuf, P->obuf, P->p2p_partial, s, P->precision == SFMM_MIXED);
+        sfmm::launch_self_correction(d, P->self, mode, nc, (which & SFMM_SELF_LIST) != 0, P->qbuf, P->obuf, s);
+    }
     HIP_TRY(hipEventRecord(ev[7], s));
     sfmm::launch_scatter_outputs(d, nc, P->obuf, dU, s);
     HIP_TRY(hipEventRecord(ev[8], s));
@@ -394,7 +500,8 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
     if (desc->m2l_kernel != 0 && desc->m2l_kernel != 1) return fail(SFMM_ERR_ARG, "m2l_kernel must be 0 or 1");
     if (desc->precision == SFMM_MIXED && desc->m2l_kernel != 0)
         return fail(SFMM_ERR_ARG, "mixed precision runs on the MFMA M2L kernel");
-    if (desc->self_row || desc->self_val) return fail(SFMM_ERR_ARG, "custom self exclusions are not available in this build");
+    if ((desc->self_row == nullptr) != (desc->self_val == nullptr))
+        return fail(SFMM_ERR_ARG, "self_row and self_val must be given together");
     if (desc->max_ncharge < 1 || desc->max_ncharge > 4) return fail(SFMM_ERR_ARG, "max_ncharge must be in 1..4");
     auto P = std::make_unique<sfmm_plan_s>();
     P->tree = tree;
@@ -461,6 +568,40 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
             ux[s] = (r[0] - c[0]) * iw;
             uy[s] = (r[1] - c[1]) * iw;
             uz[s] = (r[2] - c[2]) * iw;
+        }
+    }
+    // ---- self lists (fmm.hpp:398-408): per target, listed sources other than
+    // itself, flagged when in its near field (is_neighbor, partition.hpp:484-488)
+    std::vector<int64_t> sc_tgt, sc_row{0}, sc_src;
+    std::vector<int8_t> sc_near;
+    if (desc->self_row) {
+        const int64_t* row = desc->self_row;
+        const int64_t* val = desc->self_val;
+        if (row[0] != 0) return fail(SFMM_ERR_ARG, "self_row[0] must be 0");
+        std::vector<int32_t> slot_leaf(n);
+        for (size_t g = 0; g < lf_begin.size(); ++g)
+            for (int64_t s = lf_begin[g]; s < lf_end[g]; ++s) slot_leaf[s] = static_cast<int32_t>(g);
+        for (int64_t i = 0; i < n; ++i) {
+            if (row[i + 1] < row[i]) return fail(SFMM_ERR_ARG, "self_row must be non-decreasing");
+            const int64_t ti = iperm[i];
+            const int32_t g = slot_leaf[ti];
+            bool any = false;
+            for (int64_t k = row[i]; k < row[i + 1]; ++k) {
+                const int64_t j = val[k];
+                if (j < 0 || j >= n) return fail(SFMM_ERR_ARG, "self_val entry outside [0, n)");
+                if (j == i) continue;  // never part of the near sum
+                const int64_t sj = iperm[j];
+                bool near = false;
+                for (int64_t r = t.near_row[g]; r < t.near_row[g + 1] && !near; ++r)
+                    near = t.near_begin[r] <= sj && sj < t.near_end[r];
+                sc_src.push_back(sj);
+                sc_near.push_back(near ? 1 : 0);
+                any = true;
+            }
+            if (any) {
+                sc_tgt.push_back(ti);
+                sc_row.push_back(static_cast<int64_t>(sc_src.size()));
+            }
         }
     }
     // ---- M2L work: all nodes at levels >= 2
@@ -556,6 +697,10 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
     if ((rc = P->upload(vec, &d.field)) != SFMM_OK) return rc
     UP(px, px); UP(py, py); UP(pz, pz); UP(ux, ux); UP(uy, uy); UP(uz, uz);
     UP(t.perm, perm); UP(iperm, iperm);
+    P->self.n_row = static_cast<int64_t>(sc_tgt.size());
+    if ((rc = P->upload(sc_tgt, &P->self.tgt)) || (rc = P->upload(sc_row, &P->self.row)) ||
+        (rc = P->upload(sc_src, &P->self.src)) || (rc = P->upload(sc_near, &P->self.near)))
+        return rc;
     UP(fl_gnode, fl_gnode); UP(fl_level, fl_level); UP(fl_begin, fl_begin); UP(fl_end, fl_end);
     UP(lf_begin, lf_begin); UP(lf_end, lf_end);
     UP(t.near_row, near_row); UP(t.near_begin, near_begin); UP(t.near_end, near_end); UP(near_pref, near_pref);
@@ -581,6 +726,25 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
             if ((rc = P->upload(rotA, &P->rotA64)) || (rc = P->upload(coaxA, &P->coaxA64))) return rc;
         }
         if ((rc = P->upload(std::vector<double>(4 * P->K, 0.0), &P->zero_cols))) return rc;
+    }
+    {
+        // matrix-core M2M / L2L operators from the row-major per-octant tables
+        const int K = P->K;
+        for (int up = 0; up < 2; ++up) {
+            const std::vector<double>& R = up ? tb.l2l : tb.m2m;
+            const int nf = sfmm::translate_nfrag(P->p, up);
+            std::vector<double> frag(static_cast<size_t>(8) * nf * 64, 0.0);
+            for (int o = 0; o < 8; ++o)
+                for (int r = 0; r < K; ++r)
+                    for (int c = 0; c < K; ++c) {
+                        int lane = 0;
+                        const int f = sfmm::translate_frag_index(P->p, up, r, c, &lane);
+                        const double v = R[(static_cast<size_t>(o) * K + r) * K + c];
+                        if (f >= 0) frag[(static_cast<size_t>(o) * nf + f) * 64 + lane] = v;
+                        else if (v != 0.0) return fail(SFMM_ERR_STATE, "translation operator outside its band");
+                    }
+            if ((rc = P->upload(frag, up ? &P->l2l_frag : &P->m2m_frag))) return rc;
+        }
     }
     *out = P.release();
     return SFMM_OK;

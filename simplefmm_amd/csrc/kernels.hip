@@ -600,6 +600,33 @@ __global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, P2PItems w, const do
     }
 }
 
+// Self exclusions (fmm.hpp:786-807): the near field above skips only j == i,
+// so a listed source is subtracted when the reference would not have added it:
+// far sources always (the far field included them, _slist), near sources when
+// the exclusion predicate is the list itself.  FP64.
+template <int MODE, int NCQ, int NOUT>
+__global__ __launch_bounds__(64) void self_correction_kernel(DevPlan d, SelfCorr c, int use_near,
+                                                             const double* __restrict__ q, double* __restrict__ out)
+{
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * 64 + threadIdx.x;
+    if (r >= c.n_row) return;
+    const int64_t t = c.tgt[r];
+    const double xi = d.px[t], yi = d.py[t], zi = d.pz[t];
+    double a[NOUT];
+#pragma unroll
+    for (int k = 0; k < NOUT; ++k) a[k] = 0.0;
+    for (int64_t k = c.row[r]; k < c.row[r + 1]; ++k) {
+        if (c.near[k] && !use_near) continue;
+        const int64_t j = c.src[k];
+        double qj[NCQ];
+#pragma unroll
+        for (int cc = 0; cc < NCQ; ++cc) qj[cc] = q[cc * d.n + j];
+        pair_kernel<MODE, NCQ>(xi - d.px[j], yi - d.py[j], zi - d.pz[j], qj, a);
+    }
+#pragma unroll
+    for (int k = 0; k < NOUT; ++k) out[k * d.n + t] -= a[k];
+}
+
 // Adds the partial sums of split leaves in item order.
 __global__ void p2p_reduce_kernel(DevPlan d, P2PItems w, int nout, const double* __restrict__ partial,
                                   double* __restrict__ out)
@@ -799,6 +826,26 @@ void launch_p2p(const DevPlan& d, const P2PItems& w, Mode mode, int nc, const do
     if (w.n_item == 0) return;
     if (mixed) launch_p2p_t<true>(d, w, mode, nc, q, out, partial, s);
     else launch_p2p_t<false>(d, w, mode, nc, q, out, partial, s);
+}
+
+void launch_self_correction(const DevPlan& d, const SelfCorr& c, Mode mode, int nc, bool use_near, const double* q,
+                            double* out, hipStream_t s)
+{
+    if (c.n_row == 0) return;
+    const dim3 grid(grid_for(c.n_row, 64)), blk(64);
+    const int un = use_near ? 1 : 0;
+    if (mode == MODE_POTGRAD) {
+        hipLaunchKernelGGL((self_correction_kernel<MODE_POTGRAD, 1, 4>), grid, blk, 0, s, d, c, un, q, out);
+    } else if (mode == MODE_CURL) {
+        hipLaunchKernelGGL((self_correction_kernel<MODE_CURL, 3, 3>), grid, blk, 0, s, d, c, un, q, out);
+    } else {
+        switch (nc) {
+        case 1: hipLaunchKernelGGL((self_correction_kernel<MODE_POT, 1, 1>), grid, blk, 0, s, d, c, un, q, out); break;
+        case 2: hipLaunchKernelGGL((self_correction_kernel<MODE_POT, 2, 2>), grid, blk, 0, s, d, c, un, q, out); break;
+        case 3: hipLaunchKernelGGL((self_correction_kernel<MODE_POT, 3, 3>), grid, blk, 0, s, d, c, un, q, out); break;
+        default: hipLaunchKernelGGL((self_correction_kernel<MODE_POT, 4, 4>), grid, blk, 0, s, d, c, un, q, out); break;
+        }
+    }
 }
 
 int64_t direct_scratch_size(int64_t ntarget, int nout, int nsplit) { return ntarget * nout * nsplit; }

@@ -85,8 +85,30 @@ struct P2PItems {
     const int64_t* split_stride = nullptr;
 };
 
+// Self exclusions from a self generator (fmm.hpp:398-408): per target slot,
+// the sources (slots) of its list other than itself and whether each lies in
+// the target's near field (partition.hpp:484-488).
+struct SelfCorr {
+    int64_t n_row = 0;
+    const int64_t* tgt = nullptr;  // target slot per row
+    const int64_t* row = nullptr;  // [n_row+1]
+    const int64_t* src = nullptr;  // source slot
+    const int8_t* near = nullptr;  // 1: the source is in the target's near field
+};
+
 struct LevelSpan {
     int64_t first = 0, count = 0;  // gnode range of one level
+};
+
+// M2M / L2L on matrix cores: blocks of 16/nc output nodes, each with entries
+// (child octant, 16/nc input nodes); the operator of an octant is a dense
+// (p+1)^2 x (p+1)^2 matrix stored as MFMA A fragments.
+struct TransArgs {
+    const int64_t* blk_row = nullptr;  // [n_block+1] entry ranges
+    const int32_t* blk_out = nullptr;  // [n_block][16/nc] output gnodes, -1 = none
+    const int32_t* ent_oct = nullptr;  // child octant per entry
+    const int32_t* ent_in = nullptr;   // [entries][16/nc] input gnodes, -1 = none
+    const double* frag = nullptr;      // [8][nfrag][64]
 };
 
 // All launches are asynchronous on `stream`.
@@ -100,10 +122,19 @@ void launch_l2p(const DevPlan& d, int nc, const double* L, double* out, hipStrea
 void launch_m2l_mfma(const M2LArgs& a, int p, int nc, const double* M, double* L, hipStream_t s);
 int m2l_rot_nfrag(int p);
 int m2l_coax_nfrag(int p);
+// upper = false: M2M (parents written); true: L2L (children added to).
+void launch_translate(const TransArgs& a, LevelSpan blocks, int p, int nc, bool upper, double* io, hipStream_t s);
+int translate_nfrag(int p, bool upper);
+// A-fragment index of element (row, col) of a (p+1)^2 operator, -1 if not stored.
+int translate_frag_index(int p, bool upper, int row, int col, int* lane);
 // mixed: FP32 pair arithmetic with FP64 accumulation per source tile.
 // partial: P2PItems::n_partial_rows x nout doubles of scratch.
 void launch_p2p(const DevPlan& d, const P2PItems& w, Mode mode, int nc, const double* q, double* out,
                 double* partial, hipStream_t s, bool mixed);
+// Subtracts the pairs of the self lists: far ones always, near ones when
+// use_near (the near field then used the lists as its exclusion predicate).
+void launch_self_correction(const DevPlan& d, const SelfCorr& c, Mode mode, int nc, bool use_near, const double* q,
+                            double* out, hipStream_t s);
 void launch_direct(const DevPlan& d, Mode mode, int nc, const double* q, int64_t ntarget, const int64_t* targets,
                    double* out, double* partial, int nsplit, hipStream_t s);
 void launch_fill(double* x, int64_t count, double v, hipStream_t s);

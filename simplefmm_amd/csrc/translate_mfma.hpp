@@ -1,0 +1,188 @@
+// M2M and L2L on matrix cores.  Both shifts are level-independent dense
+// operators in the scaled real form, one per child octant (tables.cpp):
+//   M2M (fmm.hpp:558-590): output degree j reads input degrees <= j ("lower")
+//   L2L (fmm.hpp:714-745): output degree j reads input degrees >= j ("upper")
+// A workgroup stacks 16 columns (16/nc nodes x nc components); for every entry
+// (octant, 16/nc input nodes) it gathers the inputs into LDS and adds the
+// octant's operator times them into FP64 MFMA accumulators, then writes the
+// outputs once (M2M: parents are written, L2L: children are added to).
+#pragma once
+
+#include <type_traits>
+
+#include "kernels.hpp"
+#include "m2l_sched.hpp"
+
+namespace sfmm {
+
+__host__ __device__ constexpr int tr_K(int P) { return (P + 1) * (P + 1); }
+__host__ __device__ constexpr int tr_deg(int r)  // degree of real-form row r
+{
+    int j = 0;
+    while ((j + 1) * (j + 1) <= r) ++j;
+    return j;
+}
+__host__ __device__ constexpr int tr_mts(int P) { return ceil_div(tr_K(P), 16); }
+// k-step range [kb, ke) of m-tile mt: the input rows its output rows can read
+__host__ __device__ constexpr int tr_kb(int P, bool upper, int mt)
+{
+    return upper ? (tr_deg(16 * mt) * tr_deg(16 * mt)) / 4 : 0;
+}
+__host__ __device__ constexpr int tr_ke(int P, bool upper, int mt)
+{
+    const int last = 16 * mt + 15 < tr_K(P) - 1 ? 16 * mt + 15 : tr_K(P) - 1;
+    return upper ? ceil_div(tr_K(P), 4) : ceil_div((tr_deg(last) + 1) * (tr_deg(last) + 1), 4);
+}
+__host__ __device__ constexpr int tr_frag(int P, bool upper, int mt)
+{
+    int f = 0;
+    for (int q = 0; q < mt; ++q) f += tr_ke(P, upper, q) - tr_kb(P, upper, q);
+    return f;
+}
+__host__ __device__ constexpr int tr_nfrag(int P, bool upper) { return tr_frag(P, upper, tr_mts(P)); }
+
+constexpr int kTrWaves = 4;
+
+struct TrTiles {
+    int count[kTrWaves] = {};
+    int mt[kTrWaves][24] = {};
+};
+
+// m-tiles over the waves: longest k-range first, each to the least loaded wave
+template <int P, bool UPPER>
+__host__ __device__ constexpr TrTiles tr_schedule()
+{
+    TrTiles s;
+    int load[kTrWaves] = {};
+    bool done[64] = {};
+    for (int it = 0; it < tr_mts(P); ++it) {
+        int best = -1, cost = -1;
+        for (int mt = 0; mt < tr_mts(P); ++mt)
+            if (!done[mt] && tr_ke(P, UPPER, mt) - tr_kb(P, UPPER, mt) > cost) {
+                best = mt;
+                cost = tr_ke(P, UPPER, mt) - tr_kb(P, UPPER, mt);
+            }
+        done[best] = true;
+        int w = 0;
+        for (int v = 1; v < kTrWaves; ++v)
+            if (load[v] < load[w]) w = v;
+        load[w] += cost;
+        s.mt[w][s.count[w]++] = best;
+    }
+    return s;
+}
+
+namespace {
+
+constexpr int kTrLdsW = 17;  // LDS row stride: 16 columns + 1 to spread banks
+constexpr int kTrPad = 4;    // zero rows read by the last k-step
+
+template <int I, int N, class F>
+__device__ __forceinline__ void tr_static_for(F&& f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        tr_static_for<I + 1, N>(f);
+    }
+}
+
+template <int P, int NC, bool UPPER, int W>
+__device__ __forceinline__ void translate_wave(const TransArgs& a, int64_t blk, double* __restrict__ io, double* Xs)
+{
+    typedef double V4 __attribute__((ext_vector_type(4)));
+    constexpr int K = tr_K(P);
+    constexpr int TB = 16 / NC;
+    constexpr int NF = tr_nfrag(P, UPPER);
+    constexpr TrTiles S = tr_schedule<P, UPPER>();
+    constexpr int NT = S.count[W];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int col = lane & 15;
+    V4 acc[NT > 0 ? NT : 1];
+#pragma unroll
+    for (int q = 0; q < (NT > 0 ? NT : 1); ++q) acc[q] = V4{0, 0, 0, 0};
+
+    const int64_t e0 = a.blk_row[blk], e1 = a.blk_row[blk + 1];
+    for (int64_t e = e0; e < e1; ++e) {
+        const int oct = a.ent_oct[e];
+        __syncthreads();  // the previous entry's MFMAs are done with Xs
+        for (int it = tid; it < 16 * K; it += 64 * kTrWaves) {
+            const int c16 = it / K, r = it - (it / K) * K;
+            const int t = c16 / NC, c = c16 - (c16 / NC) * NC;
+            double v = 0.0;
+            if (t < TB) {
+                const int32_t src = a.ent_in[e * TB + t];
+                if (src >= 0) v = io[(static_cast<int64_t>(src) * NC + c) * K + r];
+            }
+            Xs[r * kTrLdsW + c16] = v;
+        }
+        __syncthreads();
+        const double* A = a.frag + static_cast<int64_t>(oct) * NF * 64;
+        tr_static_for<0, NT>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int mt = S.mt[W][q];
+            constexpr int kb = tr_kb(P, UPPER, mt), ke = tr_ke(P, UPPER, mt), f0 = tr_frag(P, UPPER, mt);
+#pragma unroll 4
+            for (int ks = kb; ks < ke; ++ks)
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(f0 + ks - kb) * 64 + lane],
+                                                             Xs[(4 * ks + (lane >> 4)) * kTrLdsW + col], acc[q], 0, 0,
+                                                             0);
+        });
+    }
+    const int t = col / NC, c = col - (col / NC) * NC;
+    if (t >= TB) return;
+    const int32_t dst = a.blk_out[blk * TB + t];
+    if (dst < 0) return;
+    double* out = io + (static_cast<int64_t>(dst) * NC + c) * K;
+    tr_static_for<0, NT>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int mt = S.mt[W][q];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * mt + 4 * r + (lane >> 4);  // f64 16x16x4 D layout
+            if (row < K) {
+                if constexpr (UPPER) out[row] += acc[q][r];
+                else out[row] = acc[q][r];
+            }
+        }
+    });
+}
+
+template <int P, int NC, bool UPPER>
+__global__ __launch_bounds__(64 * kTrWaves) void translate_kernel(TransArgs a, int64_t blk0, double* __restrict__ io)
+{
+    constexpr int K = tr_K(P);
+    __shared__ double Xs[(K + kTrPad) * kTrLdsW];
+    const int64_t blk = blk0 + blockIdx.x;
+    for (int i = threadIdx.x; i < kTrPad * kTrLdsW; i += 64 * kTrWaves) Xs[K * kTrLdsW + i] = 0.0;
+    switch (threadIdx.x >> 6) {
+    case 0: translate_wave<P, NC, UPPER, 0>(a, blk, io, Xs); break;
+    case 1: translate_wave<P, NC, UPPER, 1>(a, blk, io, Xs); break;
+    case 2: translate_wave<P, NC, UPPER, 2>(a, blk, io, Xs); break;
+    default: translate_wave<P, NC, UPPER, 3>(a, blk, io, Xs); break;
+    }
+}
+
+template <int P, bool UPPER>
+void translate_nc(const TransArgs& a, LevelSpan b, int nc, double* io, hipStream_t s)
+{
+    const dim3 grid(static_cast<unsigned>(b.count)), blk(64 * kTrWaves);
+    switch (nc) {
+    case 1: hipLaunchKernelGGL((translate_kernel<P, 1, UPPER>), grid, blk, 0, s, a, b.first, io); break;
+    case 2: hipLaunchKernelGGL((translate_kernel<P, 2, UPPER>), grid, blk, 0, s, a, b.first, io); break;
+    case 3: hipLaunchKernelGGL((translate_kernel<P, 3, UPPER>), grid, blk, 0, s, a, b.first, io); break;
+    default: hipLaunchKernelGGL((translate_kernel<P, 4, UPPER>), grid, blk, 0, s, a, b.first, io); break;
+    }
+}
+
+template <bool UPPER, int... Ps>
+bool translate_dispatch(const TransArgs& a, LevelSpan b, int p, int nc, double* io, hipStream_t s)
+{
+    bool done = false;
+    ((p == Ps ? (translate_nc<Ps, UPPER>(a, b, nc, io, s), done = true) : false), ...);
+    return done;
+}
+
+}  // namespace
+
+}  // namespace sfmm

@@ -165,3 +165,66 @@ def test_mixed_precision_close_to_fp64(p):
     err_phi, err_grad = rel_l2(b[:, 0], a[:, 0]), rel_l2(b[:, 1:], a[:, 1:])
     print(f"mixed vs fp64 p={p}: phi {err_phi:.3e} grad {err_grad:.3e}", flush=True)
     assert err_phi < 1e-6 and err_grad < 1e-6
+
+
+def _self_lists(pos, rng):
+    """self_generator(i) = {i, nearest neighbour of i, one random particle}."""
+    n = len(pos)
+    d2 = ((pos[:, None, :] - pos[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(d2, np.inf)
+    nn = np.argmin(d2, axis=1)
+    rnd = rng.integers(0, n, n)
+    row, val = [0], []
+    for i in range(n):
+        s = sorted({i, int(nn[i]), int(rnd[i])})
+        val.extend(s)
+        row.append(len(val))
+    return np.array(row, np.int64), np.array(val, np.int64)
+
+
+@pytest.mark.parametrize("m2l_kernel", [0, 1], ids=["mfma", "rows"])
+def test_self_lists_match_oracle(m2l_kernel):
+    # rinv(Q, U, is_self) with is_self = membership in self_generator (fmm.hpp:398-408,786-807)
+    pos, q = W.clusters(3000, seed=48)
+    rng = np.random.default_rng(5)
+    lists = _self_lists(pos, rng)
+    part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, 24)
+    fmm = api.FMM3D(part, 6, ncharge=3, m2l_kernel=m2l_kernel, self_lists=lists)
+    ot = O.Tree(pos, 20, 24)
+    for which, owhich in ((api.BOTH, O.BOTH), (api.NEAR, O.NEAR)):
+        U = np.zeros((3000, 4))
+        fmm.rinv_potgrad(q, U, which | api.SELF_LIST)
+        assert rel_l2(U, ot.fmm(6, q, O.POTGRAD, which=owhich, self_lists=lists)) < PARITY
+    Q3 = rng.uniform(-1, 1, (3000, 3))
+    F = np.zeros((3000, 3))
+    fmm.rinv(Q3, F, gradient=True, which=api.BOTH | api.SELF_LIST)
+    assert rel_l2(F, ot.fmm(6, Q3, O.CURL, self_lists=lists)) < PARITY
+
+
+def test_self_lists_default_predicate_subtracts_far_pairs_only():
+    # rinv(Q, U) on a plan built with a self generator: the near field skips j == i
+    # only, and the generator's far pairs (_slist) are subtracted
+    pos, q = W.uniform_cube(2000, seed=49)
+    rng = np.random.default_rng(6)
+    row, val = _self_lists(pos, rng)
+    part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, 16)
+    ot = O.Tree(pos, 20, 16)
+    with_lists = api.FMM3D(part, 5, ncharge=1, self_lists=(row, val))
+    plain = api.FMM3D(part, 5, ncharge=1)
+    a = np.zeros((2000, 4))
+    b = np.zeros((2000, 4))
+    with_lists.rinv_potgrad(q, a)
+    plain.rinv_potgrad(q, b)
+    expect = np.zeros((2000, 4))
+    n_far = 0
+    for i in range(2000):
+        for j in val[row[i]:row[i + 1]]:
+            if j == i or ot.is_neighbor(i, j):
+                continue
+            n_far += 1
+            d = pos[i] - pos[j]
+            r = np.linalg.norm(d)
+            expect[i, 0] -= q[j] / r
+            expect[i, 1:] += q[j] * d / r ** 3
+    assert n_far > 1000
+    assert np.max(np.abs((a - b) - expect)) < 1e-12 * np.max(np.abs(b))

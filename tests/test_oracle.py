@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from simplefmm_amd import workloads as W
+from simplefmm_amd import api, workloads as W
 from tests.helpers import golden_cases, load_golden, rel_l2
 
 SMALL_POTGRAD = [c for c in golden_cases("potgrad") if load_golden(c)[0]["n"] <= 20000]
@@ -162,3 +162,27 @@ def test_self_generator_far_pairs_are_subtracted():
     mask = np.ones(1500, bool)
     mask[i] = False
     assert np.array_equal(a[mask], b[mask])
+
+
+def test_is_neighbor_matches_near_lists():
+    # partition.hpp:484-488 against the tree's own reference-format near lists
+    pos, _ = W.clusters(2000, seed=26)
+    t = O.Tree(pos, 20, 16)
+    part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, 16)
+    leaf_of = {}
+    near = {}
+    for l in range(part.level() + 1):
+        L = part.level_lists(l)
+        idx_row, idx_val = L["idx_row"], L["idx_val"]
+        nl_row, nl_val = L["nlist_row"], L["nlist_val"].reshape(-1, 2)
+        for f in range(L["n_leaf"]):
+            for k in range(idx_row[f], idx_row[f + 1]):
+                leaf_of[int(idx_val[k])] = (l, f)
+            near[(l, f)] = {(int(a), int(b)) for a, b in nl_val[nl_row[f]:nl_row[f + 1]]}
+    rng = np.random.default_rng(7)
+    hits = 0
+    for i, j in zip(rng.integers(0, 2000, 4000), rng.integers(0, 2000, 4000)):
+        expect = leaf_of[int(j)] in near[leaf_of[int(i)]]
+        assert t.is_neighbor(i, j) == expect
+        hits += expect
+    assert hits > 0 and all(t.is_neighbor(i, i) for i in range(0, 2000, 97))
