@@ -54,6 +54,7 @@ struct sfmm_plan_s {
     std::vector<int64_t> level_first;  // gnode of the first node of each level (size depth+2)
     double* M = nullptr;
     double* L = nullptr;
+    float* M32 = nullptr;  // FP32 copy of M for the mixed-precision M2L
     int nc_alloc = 0;
     double* qbuf = nullptr;
     double* obuf = nullptr;
@@ -86,6 +87,7 @@ struct sfmm_plan_s {
         for (void* p : owned) (void)hipFree(p);
         (void)hipFree(M);
         (void)hipFree(L);
+        (void)hipFree(M32);
         (void)hipFree(qbuf);
         (void)hipFree(obuf);
         (void)hipFree(hQ);
@@ -143,11 +145,15 @@ int ensure_buffers(sfmm_plan_s* P, int nc, int qdim)
     if (nc > P->nc_alloc) {
         (void)hipFree(P->M);
         (void)hipFree(P->L);
+        (void)hipFree(P->M32);
         P->M = P->L = nullptr;
+        P->M32 = nullptr;
         P->nc_alloc = 0;
         const size_t bytes = sizeof(double) * static_cast<size_t>(nnode) * nc * P->K;
         if (hipMalloc(&P->M, bytes) != hipSuccess || hipMalloc(&P->L, bytes) != hipSuccess)
             return fail(SFMM_ERR_ALLOC, "hipMalloc failed (expansions)");
+        if (P->precision == SFMM_MIXED && P->m2l_kernel == 0 && hipMalloc(&P->M32, bytes / 2) != hipSuccess)
+            return fail(SFMM_ERR_ALLOC, "hipMalloc failed (FP32 multipoles)");
         P->nc_alloc = nc;
     }
     if (qdim > P->qdim_alloc) {
@@ -368,7 +374,14 @@ int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int
         }
     HIP_TRY(hipEventRecord(ev[3], s));
     if (far) {
-        if (P->m2l_kernel == 0) sfmm::launch_m2l_mfma(P->m2l[nc], P->p, nc, P->M, P->L, s);
+        if (P->m2l_kernel == 0) {
+            sfmm::M2LArgs& A = P->m2l[nc];
+            if (A.fp32) {
+                sfmm::launch_to_f32(P->M, P->M32, P->level_first.back() * nc * P->K, s);
+                A.M32 = P->M32;
+            }
+            sfmm::launch_m2l_mfma(A, P->p, nc, P->M, P->L, s);
+        }
         else sfmm::launch_m2l(d, nc, P->M, P->L, s);
     }
     HIP_TRY(hipEventRecord(ev[4], s));

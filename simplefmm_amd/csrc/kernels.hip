@@ -762,6 +762,29 @@ void launch_fill(double* x, int64_t count, double v, hipStream_t s)
     if (count > 0) hipLaunchKernelGGL(fill_kernel, dim3(grid_for(count, 256)), dim3(256), 0, s, x, count, v);
 }
 
+// y = (float) x, four elements per thread per step
+__global__ void to_f32_kernel(const double* __restrict__ x, float* __restrict__ y, int64_t count)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x * 4;
+    for (int64_t i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 4; i < count; i += stride) {
+        if (i + 3 < count) {
+            const double4 v = *reinterpret_cast<const double4*>(x + i);
+            *reinterpret_cast<float4*>(y + i) = make_float4(static_cast<float>(v.x), static_cast<float>(v.y),
+                                                            static_cast<float>(v.z), static_cast<float>(v.w));
+        } else {
+            for (int64_t k = i; k < count; ++k) y[k] = static_cast<float>(x[k]);
+        }
+    }
+}
+
+void launch_to_f32(const double* x, float* y, int64_t count, hipStream_t s)
+{
+    if (count <= 0) return;
+    const int64_t blocks = (count + 4 * 256 - 1) / (4 * 256);
+    hipLaunchKernelGGL(to_f32_kernel, dim3(static_cast<unsigned>(blocks < 65536 ? blocks : 65536)), dim3(256), 0, s,
+                       x, y, count);
+}
+
 void launch_p2m(const DevPlan& d, Mode mode, int nc, const double* q, double* M, hipStream_t s)
 {
     if (d.n_far_leaf == 0) return;

@@ -65,7 +65,8 @@ struct M2LArgs {
     const void* rotA = nullptr;        // rotation fragments [70][2][nfrag][64], double or float
     const void* coaxA = nullptr;       // coaxial fragments [28][nfrag][64]
     const double* off_cs = nullptr;    // [343][p+1] (cos, sin) of m * azimuth
-    const double* zero = nullptr;      // nc * (p+1)^2 zeros: the columns of absent sources
+    const double* zero = nullptr;      // 4 (p+1)^2 zeros: the columns of absent sources
+    const float* M32 = nullptr;        // FP32 copy of the multipoles (FP32 stages gather from it)
     int fp32 = 0;                      // 1: FP32 MFMA stages (mixed precision)
 };
 
@@ -138,6 +139,7 @@ void launch_self_correction(const DevPlan& d, const SelfCorr& c, Mode mode, int 
 void launch_direct(const DevPlan& d, Mode mode, int nc, const double* q, int64_t ntarget, const int64_t* targets,
                    double* out, double* partial, int nsplit, hipStream_t s);
 void launch_fill(double* x, int64_t count, double v, hipStream_t s);
+void launch_to_f32(const double* x, float* y, int64_t count, hipStream_t s);
 
 // Number of double slots the direct sum needs as scratch for `ntarget` targets.
 int64_t direct_scratch_size(int64_t ntarget, int nout, int nsplit);

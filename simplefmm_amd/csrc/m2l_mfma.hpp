@@ -166,6 +166,12 @@ __device__ __forceinline__ int ent_dist(int w) { return w >> 16; }
 // basic block the compiler can schedule freely.
 constexpr int kTrashRow = 0;  // relative to the end of the padded rows
 
+// (cos, sin) pair of the azimuth table, read with one LDS access
+template <typename T>
+struct alignas(2 * sizeof(T)) CsPair {
+    T c, s;
+};
+
 // B operands of stage 1 / 2 / 3 for wave W, in the same k-order as the A registers.
 template <int P, int W>
 __host__ __device__ constexpr int s3_boff(int q)
@@ -381,20 +387,21 @@ __device__ __forceinline__ void stage_rot_back(const T* a3, const T* b, const T*
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * mt + F::drow(lane, r);
                 const int rc = row <= j ? row : j;
-                const double cm = cs[2 * rc], sm = cs[2 * rc + 1];
-                const double lc = dc[q][mt][r], ls = ds[q][mt][r];
-                acc[q][mt][0][r] += lc * cm + ls * sm;
-                acc[q][mt][1][r] += ls * cm - lc * sm;
+                const CsPair<T> e = reinterpret_cast<const CsPair<T>*>(cs)[rc];
+                const T lc = dc[q][mt][r], ls = ds[q][mt][r];
+                acc[q][mt][0][r] += static_cast<double>(lc * e.c + ls * e.s);
+                acc[q][mt][1][r] += static_cast<double>(ls * e.c - lc * e.s);
             }
     });
 }
 
 // Source columns of one entry as a thread gathers them: for each of its
 // items (n, m) the pair (Re, Im) of coefficient (n, m), or (c(n,0), next) for m = 0.
-template <int GI>
+template <int GI, typename S>
 struct Cols {
-    double x[GI], y[GI];
+    S x[GI], y[GI];
 };
+
 
 // Gather role and operator registers of a wave.
 template <typename T, int P, int NC, int W>
@@ -439,18 +446,21 @@ struct WaveState {
     // Columns of the entry with source row srow: one 16-byte load per item
     // (the real-form pair of (n, m) is adjacent; for m = 0 the second value is
     // the next coefficient, or the zero padding after the last one).
-    __device__ __forceinline__ void load_cols(Cols<GI>& c, const M2LArgs& a, const double* __restrict__ M,
+    // S: element type of the source expansions (double, or the FP32 copy).
+    template <typename S>
+    __device__ __forceinline__ void load_cols(Cols<GI, S>& c, const M2LArgs& a, const S* __restrict__ M,
                                               const int* srow) const
     {
-        struct alignas(8) D2 {
-            double x, y;
+        struct alignas(sizeof(S)) P2 {
+            S x, y;
         };
         const int32_t src = srow[gts];
-        const double* Ms = src < 0 ? a.zero + gcc * K : M + (static_cast<int64_t>(src) * NC + gcc) * K;
+        const S* Ms = src < 0 ? reinterpret_cast<const S*>(a.zero) + gcc * K
+                              : M + (static_cast<int64_t>(src) * NC + gcc) * K;
 #pragma unroll
         for (int g = 0; g < GI; ++g) {
             const int n = gnm[g] & 0xff, m = gnm[g] >> 8;
-            const D2 v = *reinterpret_cast<const D2*>(Ms + rfc2(n, m));
+            const P2 v = *reinterpret_cast<const P2*>(Ms + rfc2(n, m));
             c.x[g] = v.x;
             c.y[g] = v.y;
         }
@@ -458,18 +468,21 @@ struct WaveState {
 
     // Columns rotated by e^{i m phi} (cos, sin from cs) into X (degree-major);
     // items this thread does not own, and the sine half of m = 0, go to the trash row.
-    __device__ __forceinline__ void store_cols(T* X, const Cols<GI>& c, const T* cs) const
+    // (the rotation is computed in the stage precision T)
+    template <typename S>
+    __device__ __forceinline__ void store_cols(T* X, const Cols<GI, S>& c, const T* cs) const
     {
         constexpr int trash = K + kPadRows + kTrashRow;
 #pragma unroll
         for (int g = 0; g < GI; ++g) {
             const int n = gnm[g] & 0xff, m = gnm[g] >> 8;
-            const double cm = cs[2 * m], sm = cs[2 * m + 1];
-            const double y = m > 0 ? c.y[g] : 0.0;
+            const CsPair<T> e = reinterpret_cast<const CsPair<T>*>(cs)[m];
+            const T x = static_cast<T>(c.x[g]);
+            const T y = m > 0 ? static_cast<T>(c.y[g]) : T(0);
             const int rc = gv[g] ? ps_row(n, m, 0) : trash;
             const int rs = gv[g] && m > 0 ? ps_row(n, m, 1) : trash;
-            X[rc * kLdsW + gc] = static_cast<T>(c.x[g] * cm - y * sm);
-            X[rs * kLdsW + gc] = static_cast<T>(c.x[g] * sm + y * cm);
+            X[rc * kLdsW + gc] = x * e.c - y * e.s;
+            X[rs * kLdsW + gc] = x * e.s + y * e.c;
         }
     }
 
@@ -561,10 +574,25 @@ __device__ __forceinline__ void write_locals(const M2LArgs& a, int64_t blk, doub
 // zeroed or repeated data with zero azimuth factors, so they add nothing.
 constexpr int kCsSlots = 8;  // azimuth table slots in LDS, by entry index mod 8
 
-// Column prefetch distance: two entries where the registers allow it (FP32
-// stages), one for FP64 (a second set would spill at p = 12).
+// Column prefetch distance: two entries for FP32 stages (which gather from
+// the FP32 copy of the multipoles), one for FP64 (a second set of FP64
+// columns would spill at p = 12).
 template <typename T>
 constexpr int kAhead = sizeof(T) == 4 ? 2 : 1;
+
+// Source expansions the kernel gathers from: M itself, or its FP32 copy.
+template <typename T>
+__device__ __forceinline__ const T* m2l_source(const M2LArgs& a, const double* M);
+template <>
+__device__ __forceinline__ const double* m2l_source<double>(const M2LArgs&, const double* M)
+{
+    return M;
+}
+template <>
+__device__ __forceinline__ const float* m2l_source<float>(const M2LArgs& a, const double*)
+{
+    return a.M32;
+}
 
 template <typename T, int P, int NC>
 __host__ __device__ constexpr size_t pipe_lds_bytes()
@@ -602,9 +630,11 @@ __device__ __forceinline__ void m2l_pipe_wave(const M2LArgs& a, int64_t blk, int
     WS ws;
     ws.init(tid);
     constexpr int AH = kAhead<T>;
-    Cols<WS::GI> c0, c1;  // columns of the entries ahead (c1 unused with one)
-    ws.load_cols(c0, a, M, ent_s + ce(0) * TB);
-    if constexpr (AH == 2) ws.load_cols(c1, a, M, ent_s + ce(1) * TB);
+    const T* Msrc = m2l_source<T>(a, M);
+    Cols<WS::GI, T> c0, c1, c2;  // columns of the entries ahead (sets beyond AH unused)
+    ws.load_cols(c0, a, Msrc, ent_s + ce(0) * TB);
+    if constexpr (AH >= 2) ws.load_cols(c1, a, Msrc, ent_s + ce(1) * TB);
+    if constexpr (AH >= 3) ws.load_cols(c2, a, Msrc, ent_s + ce(2) * TB);
     double cs_next = 0.0;
     if constexpr (W == 0) {
         const double v = load_cs<P>(a, ent_w[0], lane);
@@ -621,7 +651,7 @@ __device__ __forceinline__ void m2l_pipe_wave(const M2LArgs& a, int64_t blk, int
     }
     __syncthreads();
 
-    auto phase = [&](int i, Cols<WS::GI>& cols) {
+    auto phase = [&](int i, Cols<WS::GI, T>& cols) {
         // ---- columns of entry i (loaded two phases ago); azimuth table of entry i+1
         ws.store_cols(G + (i & 1) * BS, cols, cs_slot(i));
         if constexpr (W == 0) {  // lanes past the table write the trash tail
@@ -631,7 +661,7 @@ __device__ __forceinline__ void m2l_pipe_wave(const M2LArgs& a, int64_t blk, int
         }
         ws.use_loaded();
         // ---- loads: columns of entry i+AH, operators of entries i, i-1, i-2
-        ws.load_cols(cols, a, M, ent_s + ce(i + AH) * TB);
+        ws.load_cols(cols, a, Msrc, ent_s + ce(i + AH) * TB);
         const int w0 = ent_w[ce(i)], w1 = ent_w[ce(i - 1)], w2 = ent_w[ce(i - 2)], w3 = ent_w[ce(i - 3)];
         const int k0 = __builtin_amdgcn_readfirstlane(ent_class(w0));
         const int k1 = __builtin_amdgcn_readfirstlane(ent_class(w1));
@@ -652,9 +682,15 @@ __device__ __forceinline__ void m2l_pipe_wave(const M2LArgs& a, int64_t blk, int
         stage_coax<T, P, W>(ws.a2, in.b2, Z + (i & 1) * BS, lane);
         __syncthreads();
     };
-    // ne + 3 phases drain the pipeline; with two column sets one more phase
-    // (even count) adds nothing
-    if constexpr (AH == 2) {
+    // ne + 3 phases drain the pipeline; with AH column sets, up to AH - 1
+    // more phases (a multiple of AH) add nothing
+    if constexpr (AH == 3) {
+        for (int i = 0; i < ne + 3; i += 3) {
+            phase(i, c0);
+            phase(i + 1, c1);
+            phase(i + 2, c2);
+        }
+    } else if constexpr (AH == 2) {
         for (int i = 0; i < ne + 3; i += 2) {
             phase(i, c0);
             phase(i + 1, c1);
@@ -696,7 +732,7 @@ __global__ __launch_bounds__(kThreads, 1) void m2l_pipe_kernel(M2LArgs a, const 
     constexpr int BS = (K + kPadRows + 1) * kLdsW;
     constexpr int NCST = kCsSlots * 2 * (P + 1) + 64;
     __shared__ T buf[pipe_fits<T, P, NC>() ? 6 * BS : 1];
-    __shared__ T cst[NCST];
+    __shared__ alignas(16) T cst[NCST];
     __shared__ int ent_w[kMaxEntries];
     __shared__ int ent_s[kMaxEntries * TB];
     if constexpr (pipe_fits<T, P, NC>()) {
@@ -743,8 +779,9 @@ __device__ __forceinline__ void m2l_seq_wave(const M2LArgs& a, int64_t blk, int 
     acc.zero();
     WS ws;
     ws.init(tid);
-    Cols<WS::GI> cols;
-    ws.load_cols(cols, a, M, ent_s);
+    const T* Msrc = m2l_source<T>(a, M);
+    Cols<WS::GI, T> cols;
+    ws.load_cols(cols, a, Msrc, ent_s);
     double cs_next = 0.0;
     if constexpr (W == 0) {
         const double v = load_cs<P>(a, ent_w[0], lane);
@@ -770,7 +807,7 @@ __device__ __forceinline__ void m2l_seq_wave(const M2LArgs& a, int64_t blk, int 
         }
         ws.use_loaded();
         const int en = e + 1 < ne ? e + 1 : e;  // the last entry reloads itself, harmlessly
-        ws.load_cols(cols, a, M, ent_s + en * TB);
+        ws.load_cols(cols, a, Msrc, ent_s + en * TB);
         const int wn = ent_w[en], we = ent_w[e];
         const int cn = __builtin_amdgcn_readfirstlane(ent_class(wn));
         const int dn = __builtin_amdgcn_readfirstlane(ent_dist(wn));
@@ -804,7 +841,7 @@ __global__ __launch_bounds__(kThreads, 1) void m2l_seq_kernel(M2LArgs a, const d
     constexpr int TB = 16 / NC;
     __shared__ T Xs[(K + kPadRows + 1) * kLdsW];
     __shared__ T Ys[(K + kPadRows + 1) * kLdsW];
-    __shared__ T cst[4 * (P + 1) + 64];
+    __shared__ alignas(16) T cst[4 * (P + 1) + 64];
     __shared__ int ent_w[kMaxEntries];
     __shared__ int ent_s[kMaxEntries * TB];
     const int64_t blk = xcd_block(a.n_block);
