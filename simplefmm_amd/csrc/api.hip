@@ -731,7 +731,7 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
         return rc;
     {
         std::vector<double> rotA, coaxA;
-        sfmm::build_m2l_fragments(tb, rotA, coaxA);
+        sfmm::build_m2l_fragments(tb, P->precision == SFMM_MIXED, rotA, coaxA);
         if (P->precision == SFMM_MIXED) {
             std::vector<float> r32(rotA.begin(), rotA.end()), c32(coaxA.begin(), coaxA.end());
             if ((rc = P->upload(r32, &P->rotA32)) || (rc = P->upload(c32, &P->coaxA32))) return rc;

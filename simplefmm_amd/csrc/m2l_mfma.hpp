@@ -47,7 +47,10 @@ struct Mfma<float> {
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
-    static __device__ __forceinline__ int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
+    // The FP32 operator fragments hold operator row 4 (i % 4) + i / 4 in MFMA
+    // row i, so the expansion row held in register r of lane l is 4 r + l / 16,
+    // as for FP64: register r covers rows [4r, 4r + 4) in both.
+    static __device__ __forceinline__ int drow(int lane, int r) { return 4 * r + (lane >> 4); }
 };
 
 template <int I, int N, class F>
@@ -59,39 +62,50 @@ __device__ __forceinline__ void static_for(F&& f)
     }
 }
 
-constexpr int kWaves = 8;         // wavefronts per workgroup
-constexpr int kThreads = 64 * kWaves;
+// Schedule configuration SC = 2 * (wavefronts per workgroup) + (1 if the
+// three stages share a phase, pipelined kernel).
+__host__ __device__ constexpr int sc_waves(int SC) { return SC / 2; }
 
-template <int P>
-__host__ __device__ constexpr M2LSchedule<P, kWaves> schedule()
+template <int P, int SC>
+__host__ __device__ constexpr M2LSchedule<P, SC / 2, SC % 2 == 1> schedule()
 {
-    return M2LSchedule<P, kWaves>{};
+    return M2LSchedule<P, SC / 2, SC % 2 == 1>{};
+}
+
+// Calls f(integral_constant<W>) for the wave index w in [0, NW).
+template <int W, int NW, class F>
+__device__ __forceinline__ void for_wave(int w, F&& f)
+{
+    if constexpr (W < NW) {
+        if (w == W) f(std::integral_constant<int, W>{});
+        else for_wave<W + 1, NW>(w, f);
+    }
 }
 
 constexpr int kLdsW = 17;  // LDS row stride: 16 columns + 1 to spread banks
 constexpr int kPadRows = 8;
 
 // k-step offsets of a wave's tiles inside its register copy of the A operands
-template <int P, int W>
+template <int P, int SC, int W>
 __host__ __device__ constexpr int s1_koff(int q)
 {
-    const auto S = schedule<P>();
+    const auto S = schedule<P, SC>();
     int o = 0;
     for (int i = 0; i < q; ++i) o += rot_nks(S.s1.item[W][i] >> 8, (S.s1.item[W][i] >> 4) & 15);
     return o;
 }
-template <int P, int W>
+template <int P, int SC, int W>
 __host__ __device__ constexpr int s2_koff(int q)
 {
-    const auto S = schedule<P>();
+    const auto S = schedule<P, SC>();
     int o = 0;
     for (int i = 0; i < q; ++i) o += coax_nks(P, S.s2.item[W][i] >> 8);
     return o;
 }
-template <int P, int W>
+template <int P, int SC, int W>
 __host__ __device__ constexpr int s3_koff(int q)
 {
-    const auto S = schedule<P>();
+    const auto S = schedule<P, SC>();
     int o = 0;
     for (int i = 0; i < q; ++i) {
         const int j = S.s3.item[W][i] >> 8;
@@ -100,42 +114,42 @@ __host__ __device__ constexpr int s3_koff(int q)
     return o;
 }
 
-template <typename T, int P, int W>
+template <typename T, int P, int SC, int W>
 __device__ __forceinline__ void load_rot_fwd(T* a1, const T* A1, int lane)
 {
-    constexpr auto S = schedule<P>();
+    constexpr auto S = schedule<P, SC>();
     static_for<0, S.s1.count[W]>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int item = S.s1.item[W][q];
         constexpr int n = item >> 8, part = (item >> 4) & 15, mt = item & 15;
-        constexpr int frag = rot_frag(n, part, mt), ko = s1_koff<P, W>(q);
+        constexpr int frag = rot_frag(n, part, mt), ko = s1_koff<P, SC, W>(q);
 #pragma unroll
         for (int ks = 0; ks < rot_nks(n, part); ++ks) a1[ko + ks] = A1[(frag + ks) * 64 + lane];
     });
 }
 
-template <typename T, int P, int W>
+template <typename T, int P, int SC, int W>
 __device__ __forceinline__ void load_coax(T* a2, const T* A2, int lane)
 {
-    constexpr auto S = schedule<P>();
+    constexpr auto S = schedule<P, SC>();
     static_for<0, S.s2.count[W]>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int item = S.s2.item[W][q];
         constexpr int k = item >> 8, mt = item & 15;
-        constexpr int frag = coax_frag(P, k, mt), ko = s2_koff<P, W>(q);
+        constexpr int frag = coax_frag(P, k, mt), ko = s2_koff<P, SC, W>(q);
 #pragma unroll
         for (int ks = 0; ks < coax_nks(P, k); ++ks) a2[ko + ks] = A2[(frag + ks) * 64 + lane];
     });
 }
 
-template <typename T, int P, int W>
+template <typename T, int P, int SC, int W>
 __device__ __forceinline__ void load_rot_bwd(T* a3, const T* A3, int lane)
 {
-    constexpr auto S = schedule<P>();
+    constexpr auto S = schedule<P, SC>();
     static_for<0, S.s3.count[W]>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int j = S.s3.item[W][q] >> 8;
-        constexpr int ko = s3_koff<P, W>(q);
+        constexpr int ko = s3_koff<P, SC, W>(q);
         constexpr int nkc = rot_nks(j, 0), nks = rot_nks(j, 1);
 #pragma unroll
         for (int mt = 0; mt < rot_mts(j); ++mt) {
@@ -173,10 +187,10 @@ struct alignas(2 * sizeof(T)) CsPair {
 };
 
 // B operands of stage 1 / 2 / 3 for wave W, in the same k-order as the A registers.
-template <int P, int W>
+template <int P, int SC, int W>
 __host__ __device__ constexpr int s3_boff(int q)
 {
-    const auto S = schedule<P>();
+    const auto S = schedule<P, SC>();
     int o = 0;
     for (int i = 0; i < q; ++i) {
         const int j = S.s3.item[W][i] >> 8;
@@ -185,56 +199,56 @@ __host__ __device__ constexpr int s3_boff(int q)
     return o;
 }
 
-template <typename T, int P, int W>
+template <typename T, int P, int SC, int W>
 struct StageIn {
-    static constexpr auto S = schedule<P>();
-    static constexpr int N1 = s1_koff<P, W>(S.s1.count[W]);
-    static constexpr int N2 = s2_koff<P, W>(S.s2.count[W]);
-    static constexpr int N3 = s3_boff<P, W>(S.s3.count[W]);
+    static constexpr auto S = schedule<P, SC>();
+    static constexpr int N1 = s1_koff<P, SC, W>(S.s1.count[W]);
+    static constexpr int N2 = s2_koff<P, SC, W>(S.s2.count[W]);
+    static constexpr int N3 = s3_boff<P, SC, W>(S.s3.count[W]);
     T b1[N1 > 0 ? N1 : 1], b2[N2 > 0 ? N2 : 1], b3[N3 > 0 ? N3 : 1];
 };
 
-template <typename T, int P, int W>
+template <typename T, int P, int SC, int W>
 __device__ __forceinline__ void read_rot_fwd(T* b, const T* X, int lane)
 {
-    constexpr auto S = schedule<P>();
+    constexpr auto S = schedule<P, SC>();
     const int col = lane & 15;
     static_for<0, S.s1.count[W]>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int item = S.s1.item[W][q];
         constexpr int n = item >> 8, part = (item >> 4) & 15;
         constexpr int brow = part == 0 ? n * n : n * n + n + 1;
-        constexpr int ko = s1_koff<P, W>(q);
+        constexpr int ko = s1_koff<P, SC, W>(q);
 #pragma unroll
         for (int ks = 0; ks < rot_nks(n, part); ++ks) b[ko + ks] = X[(brow + 4 * ks + (lane >> 4)) * kLdsW + col];
     });
 }
 
-template <typename T, int P, int W>
+template <typename T, int P, int SC, int W>
 __device__ __forceinline__ void read_coax(T* b, const T* Y, int lane)
 {
-    constexpr auto S = schedule<P>();
+    constexpr auto S = schedule<P, SC>();
     const int col = lane & 15;
     static_for<0, S.s2.count[W]>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int item = S.s2.item[W][q];
         constexpr int k = item >> 8, part = (item >> 4) & 15;
         constexpr int brow = om_row(P, k, k, part);
-        constexpr int ko = s2_koff<P, W>(q);
+        constexpr int ko = s2_koff<P, SC, W>(q);
 #pragma unroll
         for (int ks = 0; ks < coax_nks(P, k); ++ks) b[ko + ks] = Y[(brow + 4 * ks + (lane >> 4)) * kLdsW + col];
     });
 }
 
-template <typename T, int P, int W>
+template <typename T, int P, int SC, int W>
 __device__ __forceinline__ void read_rot_back(T* b, const T* X, int lane)
 {
-    constexpr auto S = schedule<P>();
+    constexpr auto S = schedule<P, SC>();
     const int col = lane & 15;
     static_for<0, S.s3.count[W]>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int j = S.s3.item[W][q] >> 8;
-        constexpr int bo = s3_boff<P, W>(q);
+        constexpr int bo = s3_boff<P, SC, W>(q);
         constexpr int nkc = rot_nks(j, 0);
 #pragma unroll
         for (int ks = 0; ks < nkc; ++ks) b[bo + ks] = X[(j * j + 4 * ks + (lane >> 4)) * kLdsW + col];
@@ -245,10 +259,10 @@ __device__ __forceinline__ void read_rot_back(T* b, const T* X, int lane)
 }
 
 // Longest k-chain among a wave's tiles of each stage.
-template <int P, int W>
+template <int P, int SC, int W>
 __host__ __device__ constexpr int s1_maxks()
 {
-    const auto S = schedule<P>();
+    const auto S = schedule<P, SC>();
     int m = 0;
     for (int q = 0; q < S.s1.count[W]; ++q) {
         const int it = S.s1.item[W][q];
@@ -256,18 +270,18 @@ __host__ __device__ constexpr int s1_maxks()
     }
     return m;
 }
-template <int P, int W>
+template <int P, int SC, int W>
 __host__ __device__ constexpr int s2_maxks()
 {
-    const auto S = schedule<P>();
+    const auto S = schedule<P, SC>();
     int m = 0;
     for (int q = 0; q < S.s2.count[W]; ++q) m = coax_nks(P, S.s2.item[W][q] >> 8) > m ? coax_nks(P, S.s2.item[W][q] >> 8) : m;
     return m;
 }
-template <int P, int W>
+template <int P, int SC, int W>
 __host__ __device__ constexpr int s3_maxks()
 {
-    const auto S = schedule<P>();
+    const auto S = schedule<P, SC>();
     int m = 0;
     for (int q = 0; q < S.s3.count[W]; ++q) m = rot_nks(S.s3.item[W][q] >> 8, 0) > m ? rot_nks(S.s3.item[W][q] >> 8, 0) : m;
     return m;
@@ -277,24 +291,24 @@ __host__ __device__ constexpr int s3_maxks()
 // consecutive MFMAs are independent.
 
 // Stage 1: polar rotation per degree, degree-major X -> order-major Y.
-template <typename T, int P, int W>
+template <typename T, int P, int SC, int W>
 __device__ __forceinline__ void stage_rot_fwd(const T* a1, const T* b, T* Y, int lane)
 {
     using F = Mfma<T>;
     using V4 = typename F::V4;
-    constexpr auto S = schedule<P>();
+    constexpr auto S = schedule<P, SC>();
     constexpr int K = (P + 1) * (P + 1);
     constexpr int NQ = S.s1.count[W];
     const int col = lane & 15;
     V4 d[NQ > 0 ? NQ : 1];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) d[q] = V4{0, 0, 0, 0};
-    static_for<0, s1_maxks<P, W>()>([&](auto kc) {
+    static_for<0, s1_maxks<P, SC, W>()>([&](auto kc) {
         constexpr int ks = decltype(kc)::value;
         static_for<0, NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             constexpr int item = S.s1.item[W][q];
-            constexpr int ko = s1_koff<P, W>(q);
+            constexpr int ko = s1_koff<P, SC, W>(q);
             if constexpr (ks < rot_nks(item >> 8, (item >> 4) & 15)) d[q] = F::run(a1[ko + ks], b[ko + ks], d[q]);
         });
     });
@@ -302,34 +316,36 @@ __device__ __forceinline__ void stage_rot_fwd(const T* a1, const T* b, T* Y, int
         constexpr int q = decltype(qc)::value;
         constexpr int item = S.s1.item[W][q];
         constexpr int n = item >> 8, part = (item >> 4) & 15, mt = item & 15;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = 16 * mt + F::drow(lane, r);
-            const bool ok = row <= n && (part == 0 || row > 0);
-            const int dst = ok ? om_row(P, row, n, part) : K + kPadRows + kTrashRow;
-            Y[dst * kLdsW + col] = d[q][r];
-        }
+        static_for<0, 4>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            if constexpr (16 * mt + 4 * r <= n) {  // register r holds rows [4r, 4r + 4) of the tile
+                const int row = 16 * mt + F::drow(lane, r);
+                const bool ok = row <= n && (part == 0 || row > 0);
+                const int dst = ok ? om_row(P, row, n, part) : K + kPadRows + kTrashRow;
+                Y[dst * kLdsW + col] = d[q][r];
+            }
+        });
     });
 }
 
 // Stage 2: coaxial translation per order, order-major Y -> degree-major X.
-template <typename T, int P, int W>
+template <typename T, int P, int SC, int W>
 __device__ __forceinline__ void stage_coax(const T* a2, const T* b, T* X, int lane)
 {
     using F = Mfma<T>;
     using V4 = typename F::V4;
-    constexpr auto S = schedule<P>();
+    constexpr auto S = schedule<P, SC>();
     constexpr int K = (P + 1) * (P + 1);
     constexpr int NQ = S.s2.count[W];
     const int col = lane & 15;
     V4 d[NQ > 0 ? NQ : 1];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) d[q] = V4{0, 0, 0, 0};
-    static_for<0, s2_maxks<P, W>()>([&](auto kc) {
+    static_for<0, s2_maxks<P, SC, W>()>([&](auto kc) {
         constexpr int ks = decltype(kc)::value;
         static_for<0, NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            constexpr int ko = s2_koff<P, W>(q);
+            constexpr int ko = s2_koff<P, SC, W>(q);
             if constexpr (ks < coax_nks(P, S.s2.item[W][q] >> 8)) d[q] = F::run(a2[ko + ks], b[ko + ks], d[q]);
         });
     });
@@ -337,24 +353,26 @@ __device__ __forceinline__ void stage_coax(const T* a2, const T* b, T* X, int la
         constexpr int q = decltype(qc)::value;
         constexpr int item = S.s2.item[W][q];
         constexpr int k = item >> 8, part = (item >> 4) & 15, mt = item & 15;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int j = k + 16 * mt + F::drow(lane, r);
-            const int dst = j <= P ? ps_row(j, k, part) : K + kPadRows + kTrashRow;
-            X[dst * kLdsW + col] = d[q][r];
-        }
+        static_for<0, 4>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            if constexpr (k + 16 * mt + 4 * r <= P) {
+                const int j = k + 16 * mt + F::drow(lane, r);
+                const int dst = j <= P ? ps_row(j, k, part) : K + kPadRows + kTrashRow;
+                X[dst * kLdsW + col] = d[q][r];
+            }
+        });
     });
 }
 
 // Stage 3: back rotation per degree, azimuth e^{-i m phi} from cs, FP64
 // accumulation.  Rows beyond a degree have all-zero operator rows, so their
 // results are zero and add nothing (cs is read at a clamped, valid index).
-template <typename T, int P, int W, class Acc>
+template <typename T, int P, int SC, int W, class Acc>
 __device__ __forceinline__ void stage_rot_back(const T* a3, const T* b, const T* cs, Acc& acc, int lane)
 {
     using F = Mfma<T>;
     using V4 = typename F::V4;
-    constexpr auto S = schedule<P>();
+    constexpr auto S = schedule<P, SC>();
     constexpr int NQ = S.s3.count[W];
     constexpr int MT = rot_mts(P);
     V4 dc[NQ > 0 ? NQ : 1][MT], ds[NQ > 0 ? NQ : 1][MT];
@@ -362,13 +380,13 @@ __device__ __forceinline__ void stage_rot_back(const T* a3, const T* b, const T*
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) dc[q][mt] = ds[q][mt] = V4{0, 0, 0, 0};
-    static_for<0, s3_maxks<P, W>()>([&](auto kc) {
+    static_for<0, s3_maxks<P, SC, W>()>([&](auto kc) {
         constexpr int ks = decltype(kc)::value;
         static_for<0, NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             constexpr int j = S.s3.item[W][q] >> 8;
-            constexpr int ko = s3_koff<P, W>(q);
-            constexpr int bo = s3_boff<P, W>(q);
+            constexpr int ko = s3_koff<P, SC, W>(q);
+            constexpr int bo = s3_boff<P, SC, W>(q);
             constexpr int nkc = rot_nks(j, 0), nks = rot_nks(j, 1);
             static_for<0, rot_mts(j)>([&](auto mc) {
                 constexpr int mt = decltype(mc)::value;
@@ -381,17 +399,20 @@ __device__ __forceinline__ void stage_rot_back(const T* a3, const T* b, const T*
     static_for<0, NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int j = S.s3.item[W][q] >> 8;
-#pragma unroll
-        for (int mt = 0; mt < rot_mts(j); ++mt)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = 16 * mt + F::drow(lane, r);
-                const int rc = row <= j ? row : j;
-                const CsPair<T> e = reinterpret_cast<const CsPair<T>*>(cs)[rc];
-                const T lc = dc[q][mt][r], ls = ds[q][mt][r];
-                acc[q][mt][0][r] += static_cast<double>(lc * e.c + ls * e.s);
-                acc[q][mt][1][r] += static_cast<double>(ls * e.c - lc * e.s);
-            }
+        static_for<0, rot_mts(j)>([&](auto mc) {
+            constexpr int mt = decltype(mc)::value;
+            static_for<0, 4>([&](auto rc_) {
+                constexpr int r = decltype(rc_)::value;
+                if constexpr (16 * mt + 4 * r <= j) {  // registers past the degree hold zeros
+                    const int row = 16 * mt + F::drow(lane, r);
+                    const int rc = row <= j ? row : j;
+                    const CsPair<T> e = reinterpret_cast<const CsPair<T>*>(cs)[rc];
+                    const T lc = dc[q][mt][r], ls = ds[q][mt][r];
+                    acc[q][mt][0][r] += static_cast<double>(lc * e.c + ls * e.s);
+                    acc[q][mt][1][r] += static_cast<double>(ls * e.c - lc * e.s);
+                }
+            });
+        });
     });
 }
 
@@ -404,19 +425,19 @@ struct Cols {
 
 
 // Gather role and operator registers of a wave.
-template <typename T, int P, int NC, int W>
+template <typename T, int P, int NC, int SC, int W>
 struct WaveState {
     static constexpr int K = (P + 1) * (P + 1);
     static constexpr int NT = (P + 1) * (P + 2) / 2;
     static constexpr int TB = 16 / NC;
     static constexpr int NRF = rot_nfrag(P);
     static constexpr int NCF = coax_nfrag(P);
-    static constexpr auto S = schedule<P>();
+    static constexpr auto S = schedule<P, SC>();
     static constexpr int NJ = S.s3.count[W];
-    static constexpr int NK1 = s1_koff<P, W>(S.s1.count[W]);
-    static constexpr int NK2 = s2_koff<P, W>(S.s2.count[W]);
-    static constexpr int NK3 = s3_koff<P, W>(NJ);
-    static constexpr int CPT = kThreads / 16;        // threads per gathered column
+    static constexpr int NK1 = s1_koff<P, SC, W>(S.s1.count[W]);
+    static constexpr int NK2 = s2_koff<P, SC, W>(S.s2.count[W]);
+    static constexpr int NK3 = s3_koff<P, SC, W>(NJ);
+    static constexpr int CPT = 4 * sc_waves(SC);     // threads per gathered column (64 NW / 16)
     static constexpr int GI = (NT + CPT - 1) / CPT;  // gather items per thread
 
     int gc, gcc, gts;
@@ -498,15 +519,15 @@ struct WaveState {
 
     __device__ __forceinline__ void load_fwd(const M2LArgs& a, int cls, int lane)
     {
-        load_rot_fwd<T, P, W>(b1, static_cast<const T*>(a.rotA) + (static_cast<int64_t>(cls) * 2 + 0) * NRF * 64, lane);
+        load_rot_fwd<T, P, SC, W>(b1, static_cast<const T*>(a.rotA) + (static_cast<int64_t>(cls) * 2 + 0) * NRF * 64, lane);
     }
     __device__ __forceinline__ void load_back(const M2LArgs& a, int cls, int lane)
     {
-        load_rot_bwd<T, P, W>(b3, static_cast<const T*>(a.rotA) + (static_cast<int64_t>(cls) * 2 + 1) * NRF * 64, lane);
+        load_rot_bwd<T, P, SC, W>(b3, static_cast<const T*>(a.rotA) + (static_cast<int64_t>(cls) * 2 + 1) * NRF * 64, lane);
     }
     __device__ __forceinline__ void load_cx(const M2LArgs& a, int d2, int lane)
     {
-        load_coax<T, P, W>(b2, static_cast<const T*>(a.coaxA) + static_cast<int64_t>(d2) * NCF * 64, lane);
+        load_coax<T, P, SC, W>(b2, static_cast<const T*>(a.coaxA) + static_cast<int64_t>(d2) * NCF * 64, lane);
     }
 };
 
@@ -534,13 +555,13 @@ struct Acc {
     }
 };
 
-template <typename T, int P, int NC, int W, class A>
+template <typename T, int P, int NC, int SC, int W, class A>
 __device__ __forceinline__ void write_locals(const M2LArgs& a, int64_t blk, double* __restrict__ L, A& acc, int lane)
 {
     using F = Mfma<T>;
     constexpr int K = (P + 1) * (P + 1);
     constexpr int TB = 16 / NC;
-    constexpr auto S = schedule<P>();
+    constexpr auto S = schedule<P, SC>();
     const int col = lane & 15;
     const int t = col / NC, c = col - (col / NC) * NC;
     if (t >= TB) return;
@@ -607,12 +628,12 @@ __host__ __device__ constexpr bool pipe_fits()
     return pipe_lds_bytes<T, P, NC>() <= 160 * 1024;
 }
 
-template <typename T, int P, int NC, int W>
+template <typename T, int P, int NC, int SC, int W>
 __device__ __forceinline__ void m2l_pipe_wave(const M2LArgs& a, int64_t blk, int ne, const double* __restrict__ M,
                                               double* __restrict__ L, T* buf, T* cst, const int* ent_w,
                                               const int* ent_s)
 {
-    using WS = WaveState<T, P, NC, W>;
+    using WS = WaveState<T, P, NC, SC, W>;
     constexpr int K = (P + 1) * (P + 1);
     constexpr int BS = (K + kPadRows + 1) * kLdsW;
     constexpr int TB = 16 / NC;
@@ -673,13 +694,13 @@ __device__ __forceinline__ void m2l_pipe_wave(const M2LArgs& a, int64_t blk, int
         if (d1 != d2) ws.load_cx(a, d1, lane);
         if (k2 != k3) ws.load_back(a, k2, lane);
         // ---- the three stages on entries i-1, i-2, i-3
-        StageIn<T, P, W> in;
-        read_rot_fwd<T, P, W>(in.b1, G + ((i - 1) & 1) * BS, lane);
-        read_coax<T, P, W>(in.b2, Y + (i & 1) * BS, lane);
-        read_rot_back<T, P, W>(in.b3, Z + ((i - 1) & 1) * BS, lane);
-        stage_rot_back<T, P, W>(ws.a3, in.b3, cs_slot(i - 3), acc, lane);
-        stage_rot_fwd<T, P, W>(ws.a1, in.b1, Y + ((i - 1) & 1) * BS, lane);
-        stage_coax<T, P, W>(ws.a2, in.b2, Z + (i & 1) * BS, lane);
+        StageIn<T, P, SC, W> in;
+        read_rot_fwd<T, P, SC, W>(in.b1, G + ((i - 1) & 1) * BS, lane);
+        read_coax<T, P, SC, W>(in.b2, Y + (i & 1) * BS, lane);
+        read_rot_back<T, P, SC, W>(in.b3, Z + ((i - 1) & 1) * BS, lane);
+        stage_rot_back<T, P, SC, W>(ws.a3, in.b3, cs_slot(i - 3), acc, lane);
+        stage_rot_fwd<T, P, SC, W>(ws.a1, in.b1, Y + ((i - 1) & 1) * BS, lane);
+        stage_coax<T, P, SC, W>(ws.a2, in.b2, Z + (i & 1) * BS, lane);
         __syncthreads();
     };
     // ne + 3 phases drain the pipeline; with AH column sets, up to AH - 1
@@ -698,7 +719,7 @@ __device__ __forceinline__ void m2l_pipe_wave(const M2LArgs& a, int64_t blk, int
     } else {
         for (int i = 0; i < ne + 3; ++i) phase(i, c0);
     }
-    write_locals<T, P, NC, W>(a, blk, L, acc, lane);
+    write_locals<T, P, NC, SC, W>(a, blk, L, acc, lane);
 }
 
 // Targets with an empty interaction list get L = 0.
@@ -707,7 +728,7 @@ __device__ __forceinline__ void m2l_zero(const M2LArgs& a, int64_t blk, double* 
 {
     constexpr int K = (P + 1) * (P + 1);
     constexpr int TB = 16 / NC;
-    for (int i = threadIdx.x; i < TB * NC * K; i += kThreads) {
+    for (int i = threadIdx.x; i < TB * NC * K; i += blockDim.x) {
         const int t = i / (NC * K);
         const int32_t tg = a.blk_tgt[blk * TB + t];
         if (tg >= 0) L[static_cast<int64_t>(tg) * NC * K + (i - t * NC * K)] = 0.0;
@@ -723,10 +744,22 @@ __device__ __forceinline__ int64_t xcd_block(int64_t n_block)
     return static_cast<int64_t>(blockIdx.x % 8) * per + blockIdx.x / 8;
 }
 
-template <typename T, int P, int NC>
-__global__ __launch_bounds__(kThreads, 1) void m2l_pipe_kernel(M2LArgs a, const double* __restrict__ M,
-                                                              double* __restrict__ L)
+// Wavefronts per workgroup: 12 (3 per SIMD), measured best at p = 12 for both
+// stage precisions (FP32 against 8 and 16, FP64 against 8).  The sequential
+// kernel uses 8.
+template <typename T>
+__host__ __device__ constexpr int pipe_waves()
 {
+    return 12;
+}
+constexpr int kSeqWaves = 8;
+
+template <typename T, int P, int NC>
+__global__ __launch_bounds__(64 * pipe_waves<T>(), 1) void m2l_pipe_kernel(M2LArgs a, const double* __restrict__ M,
+                                                                          double* __restrict__ L)
+{
+    constexpr int NW = pipe_waves<T>();
+    constexpr int NTH = 64 * NW;
     constexpr int K = (P + 1) * (P + 1);
     constexpr int TB = 16 / NC;
     constexpr int BS = (K + kPadRows + 1) * kLdsW;
@@ -744,33 +777,26 @@ __global__ __launch_bounds__(kThreads, 1) void m2l_pipe_kernel(M2LArgs a, const 
             m2l_zero<P, NC>(a, blk, L);
             return;
         }
-        for (int i = threadIdx.x; i < 6 * BS; i += kThreads) buf[i] = T(0);
-        for (int i = threadIdx.x; i < NCST; i += kThreads) cst[i] = T(0);
-        for (int i = threadIdx.x; i < ne; i += kThreads) ent_w[i] = a.ent_meta[e0 + i];
-        for (int i = threadIdx.x; i < ne * TB; i += kThreads) ent_s[i] = a.ent_src[e0 * TB + i];
+        for (int i = threadIdx.x; i < 6 * BS; i += NTH) buf[i] = T(0);
+        for (int i = threadIdx.x; i < NCST; i += NTH) cst[i] = T(0);
+        for (int i = threadIdx.x; i < ne; i += NTH) ent_w[i] = a.ent_meta[e0 + i];
+        for (int i = threadIdx.x; i < ne * TB; i += NTH) ent_s[i] = a.ent_src[e0 * TB + i];
         __syncthreads();
-        switch (threadIdx.x >> 6) {
-        case 0: m2l_pipe_wave<T, P, NC, 0>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
-        case 1: m2l_pipe_wave<T, P, NC, 1>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
-        case 2: m2l_pipe_wave<T, P, NC, 2>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
-        case 3: m2l_pipe_wave<T, P, NC, 3>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
-        case 4: m2l_pipe_wave<T, P, NC, 4>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
-        case 5: m2l_pipe_wave<T, P, NC, 5>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
-        case 6: m2l_pipe_wave<T, P, NC, 6>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
-        default: m2l_pipe_wave<T, P, NC, 7>(a, blk, ne, M, L, buf, cst, ent_w, ent_s); break;
-        }
+        for_wave<0, NW>(threadIdx.x >> 6, [&](auto wc) {
+            m2l_pipe_wave<T, P, NC, 2 * NW + 1, decltype(wc)::value>(a, blk, ne, M, L, buf, cst, ent_w, ent_s);
+        });
     }
 }
 
 // ---------------------------------------------------------------- sequential kernel
 // For orders whose six pipeline buffers exceed LDS (FP64, p > 12): one entry
 // at a time, four barriers per entry, loads issued one entry ahead.
-template <typename T, int P, int NC, int W>
+template <typename T, int P, int NC, int SC, int W>
 __device__ __forceinline__ void m2l_seq_wave(const M2LArgs& a, int64_t blk, int ne, const double* __restrict__ M,
                                              double* __restrict__ L, T* Xs, T* Ys, T* cst, const int* ent_w,
                                              const int* ent_s)
 {
-    using WS = WaveState<T, P, NC, W>;
+    using WS = WaveState<T, P, NC, SC, W>;
     constexpr int TB = 16 / NC;
     constexpr int NCS = 2 * (P + 1);
     const int tid = threadIdx.x;
@@ -819,24 +845,25 @@ __device__ __forceinline__ void m2l_seq_wave(const M2LArgs& a, int64_t blk, int 
         }
         if (dn != d_e) ws.load_cx(a, dn, lane);
         __syncthreads();
-        StageIn<T, P, W> in;
-        read_rot_fwd<T, P, W>(in.b1, Xs, lane);
-        stage_rot_fwd<T, P, W>(ws.a1, in.b1, Ys, lane);
+        StageIn<T, P, SC, W> in;
+        read_rot_fwd<T, P, SC, W>(in.b1, Xs, lane);
+        stage_rot_fwd<T, P, SC, W>(ws.a1, in.b1, Ys, lane);
         __syncthreads();
-        read_coax<T, P, W>(in.b2, Ys, lane);
-        stage_coax<T, P, W>(ws.a2, in.b2, Xs, lane);
+        read_coax<T, P, SC, W>(in.b2, Ys, lane);
+        stage_coax<T, P, SC, W>(ws.a2, in.b2, Xs, lane);
         __syncthreads();
-        read_rot_back<T, P, W>(in.b3, Xs, lane);
-        stage_rot_back<T, P, W>(ws.a3, in.b3, cs, acc, lane);
+        read_rot_back<T, P, SC, W>(in.b3, Xs, lane);
+        stage_rot_back<T, P, SC, W>(ws.a3, in.b3, cs, acc, lane);
         __syncthreads();
     }
-    write_locals<T, P, NC, W>(a, blk, L, acc, lane);
+    write_locals<T, P, NC, SC, W>(a, blk, L, acc, lane);
 }
 
 template <typename T, int P, int NC>
-__global__ __launch_bounds__(kThreads, 1) void m2l_seq_kernel(M2LArgs a, const double* __restrict__ M,
-                                                             double* __restrict__ L)
+__global__ __launch_bounds__(64 * kSeqWaves, 1) void m2l_seq_kernel(M2LArgs a, const double* __restrict__ M,
+                                                                   double* __restrict__ L)
 {
+    constexpr int NTH = 64 * kSeqWaves;
     constexpr int K = (P + 1) * (P + 1);
     constexpr int TB = 16 / NC;
     __shared__ T Xs[(K + kPadRows + 1) * kLdsW];
@@ -852,23 +879,16 @@ __global__ __launch_bounds__(kThreads, 1) void m2l_seq_kernel(M2LArgs a, const d
         m2l_zero<P, NC>(a, blk, L);
         return;
     }
-    for (int i = threadIdx.x; i < kPadRows * kLdsW; i += kThreads) {
+    for (int i = threadIdx.x; i < kPadRows * kLdsW; i += NTH) {
         Xs[K * kLdsW + i] = T(0);
         Ys[K * kLdsW + i] = T(0);
     }
-    for (int i = threadIdx.x; i < ne; i += kThreads) ent_w[i] = a.ent_meta[e0 + i];
-    for (int i = threadIdx.x; i < ne * TB; i += kThreads) ent_s[i] = a.ent_src[e0 * TB + i];
+    for (int i = threadIdx.x; i < ne; i += NTH) ent_w[i] = a.ent_meta[e0 + i];
+    for (int i = threadIdx.x; i < ne * TB; i += NTH) ent_s[i] = a.ent_src[e0 * TB + i];
     __syncthreads();
-    switch (threadIdx.x >> 6) {
-    case 0: m2l_seq_wave<T, P, NC, 0>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
-    case 1: m2l_seq_wave<T, P, NC, 1>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
-    case 2: m2l_seq_wave<T, P, NC, 2>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
-    case 3: m2l_seq_wave<T, P, NC, 3>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
-    case 4: m2l_seq_wave<T, P, NC, 4>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
-    case 5: m2l_seq_wave<T, P, NC, 5>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
-    case 6: m2l_seq_wave<T, P, NC, 6>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
-    default: m2l_seq_wave<T, P, NC, 7>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s); break;
-    }
+    for_wave<0, kSeqWaves>(threadIdx.x >> 6, [&](auto wc) {
+        m2l_seq_wave<T, P, NC, 2 * kSeqWaves, decltype(wc)::value>(a, blk, ne, M, L, Xs, Ys, cst, ent_w, ent_s);
+    });
 }
 
 template <typename T, int P, int NC>
@@ -876,9 +896,9 @@ void launch_one(const M2LArgs& a, const double* M, double* L, hipStream_t s)
 {
     const unsigned grid = static_cast<unsigned>(((a.n_block + 7) / 8) * 8);
     if constexpr (pipe_fits<T, P, NC>())
-        hipLaunchKernelGGL((m2l_pipe_kernel<T, P, NC>), dim3(grid), dim3(kThreads), 0, s, a, M, L);
+        hipLaunchKernelGGL((m2l_pipe_kernel<T, P, NC>), dim3(grid), dim3(64 * pipe_waves<T>()), 0, s, a, M, L);
     else
-        hipLaunchKernelGGL((m2l_seq_kernel<T, P, NC>), dim3(grid), dim3(kThreads), 0, s, a, M, L);
+        hipLaunchKernelGGL((m2l_seq_kernel<T, P, NC>), dim3(grid), dim3(64 * kSeqWaves), 0, s, a, M, L);
 }
 
 template <typename T, int P>

@@ -56,49 +56,62 @@ __host__ __device__ constexpr int coax_frag(int P, int k, int mt)
 }
 __host__ __device__ constexpr int coax_nfrag(int P) { return coax_frag(P, P + 1, 0); }
 
-// Static work split over the NW wavefronts of a workgroup: longest tiles first,
-// each to the least loaded wave.
-constexpr int kMaxWaves = 8;
+// Static work split over the NW wavefronts of a workgroup: longest tiles
+// first, each to the least loaded wave.  PIPE: the three stages run in the same
+// phase (pipelined kernel), so the split balances each wave's total over all
+// stages; otherwise each stage is balanced on its own (barrier between stages).
+constexpr int kMaxWaves = 16;
 struct WaveTiles {
     int count[kMaxWaves] = {};
     int item[kMaxWaves][48] = {};  // packed (index << 8) | (part << 4) | mt
 };
 
-template <int P, int NW>
+template <int P, int NW, bool PIPE>
 struct M2LSchedule {
     WaveTiles s1, s2, s3;  // s3 items are degrees j (both parts, all m-tiles)
 
     constexpr M2LSchedule()
     {
-        int load[kMaxWaves] = {};
-        auto pick = [&](int cost) {
-            int w = 0;
-            for (int v = 1; v < NW; ++v)
-                if (load[v] < load[w]) w = v;
-            load[w] += cost;
-            return w;
-        };
+        // all tiles: (stage, packed item, cost)
+        int st[3 * 64] = {}, it[3 * 64] = {}, cost[3 * 64] = {};
+        int nt = 0;
         for (int n = P; n >= 0; --n)
             for (int part = 0; part < 2; ++part) {
                 if (part == 1 && n == 0) continue;
                 for (int mt = 0; mt < rot_mts(n); ++mt) {
-                    const int w = pick(rot_nks(n, part));
-                    s1.item[w][s1.count[w]++] = (n << 8) | (part << 4) | mt;
+                    st[nt] = 1;
+                    it[nt] = (n << 8) | (part << 4) | mt;
+                    cost[nt++] = rot_nks(n, part);
                 }
             }
-        for (int v = 0; v < kMaxWaves; ++v) load[v] = 0;
         for (int k = 0; k <= P; ++k)
             for (int part = 0; part < 2; ++part) {
                 if (part == 1 && k == 0) continue;
                 for (int mt = 0; mt < coax_mts(P, k); ++mt) {
-                    const int w = pick(coax_nks(P, k));
-                    s2.item[w][s2.count[w]++] = (k << 8) | (part << 4) | mt;
+                    st[nt] = 2;
+                    it[nt] = (k << 8) | (part << 4) | mt;
+                    cost[nt++] = coax_nks(P, k);
                 }
             }
-        for (int v = 0; v < kMaxWaves; ++v) load[v] = 0;
         for (int j = P; j >= 0; --j) {
-            const int w = pick(rot_mts(j) * (rot_nks(j, 0) + rot_nks(j, 1)));
-            s3.item[w][s3.count[w]++] = j << 8;
+            st[nt] = 3;
+            it[nt] = j << 8;
+            cost[nt++] = rot_mts(j) * (rot_nks(j, 0) + rot_nks(j, 1));
+        }
+        int load[3][kMaxWaves] = {};
+        bool done[3 * 64] = {};
+        for (int rep = 0; rep < nt; ++rep) {
+            int b = -1;
+            for (int t = 0; t < nt; ++t)
+                if (!done[t] && (b < 0 || cost[t] > cost[b])) b = t;
+            done[b] = true;
+            int* ld = load[PIPE ? 0 : st[b] - 1];
+            int w = 0;
+            for (int v = 1; v < NW; ++v)
+                if (ld[v] < ld[w]) w = v;
+            ld[w] += cost[b];
+            WaveTiles& T = st[b] == 1 ? s1 : st[b] == 2 ? s2 : s3;
+            T.item[w][T.count[w]++] = it[b];
         }
     }
 };

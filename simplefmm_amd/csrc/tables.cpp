@@ -110,8 +110,10 @@ void complex_to_real_form(int p, const std::vector<double>& Tre, const std::vect
         }
 }
 
-void build_m2l_fragments(const Tables& t, std::vector<double>& rotA, std::vector<double>& coaxA)
+void build_m2l_fragments(const Tables& t, bool interleave_rows, std::vector<double>& rotA,
+                         std::vector<double>& coaxA)
 {
+    auto mrow = [&](int l) { return interleave_rows ? 4 * (l % 4) + (l % 16) / 4 : l % 16; };
     const int P = t.p;
     const int nrf = rot_nfrag(P), ncf = coax_nfrag(P);
     rotA.assign(static_cast<size_t>(70) * 2 * nrf * 64, 0.0);
@@ -127,7 +129,7 @@ void build_m2l_fragments(const Tables& t, std::vector<double>& rotA, std::vector
                         for (int ks = 0; ks < rot_nks(n, part); ++ks) {
                             double* f = out + static_cast<size_t>(rot_frag(n, part, mt) + ks) * 64;
                             for (int l = 0; l < 64; ++l) {
-                                const int row = 16 * mt + l % 16, kk = 4 * ks + l / 16;
+                                const int row = 16 * mt + mrow(l), kk = 4 * ks + l / 16;
                                 double v = 0.0;
                                 if (part == 0 && row <= n && kk <= n)
                                     v = blocks[t.rot_off[n] + row * (n + 1) + kk];
@@ -147,7 +149,7 @@ void build_m2l_fragments(const Tables& t, std::vector<double>& rotA, std::vector
                 for (int ks = 0; ks < coax_nks(P, k); ++ks) {
                     double* f = out + static_cast<size_t>(coax_frag(P, k, mt) + ks) * 64;
                     for (int l = 0; l < 64; ++l) {
-                        const int row = 16 * mt + l % 16, kk = 4 * ks + l / 16;
+                        const int row = 16 * mt + mrow(l), kk = 4 * ks + l / 16;
                         f[l] = (row < w && kk < w) ? C[t.coax_off[k] + row * w + kk] : 0.0;
                     }
                 }

@@ -47,7 +47,12 @@ struct Tables {
 
 // MFMA A-operand fragments of the M2L blocks (layout in m2l_sched.hpp):
 // rotA [70 classes][2 directions][nfrag][64], coaxA [28 |d|^2][nfrag][64].
-void build_m2l_fragments(const Tables& t, std::vector<double>& rotA, std::vector<double>& coaxA);
+// Lane l of fragment (mt, ks) holds operator element (16 mt + i', 4 ks + l / 16)
+// with i' = l % 16, or, with interleave_rows (FP32 MFMA, whose output register r
+// covers MFMA rows 4 (l / 16) + r), i' = 4 (i % 4) + i / 4 for i = l % 16, so
+// that register r holds operator rows [4r, 4r + 4) as in the FP64 layout.
+void build_m2l_fragments(const Tables& t, bool interleave_rows, std::vector<double>& rotA,
+                         std::vector<double>& coaxA);
 
 // Builds all tables for expansion order p (1 <= p <= 20).
 void build_tables(int p, Tables& t);
