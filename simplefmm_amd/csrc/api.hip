@@ -583,6 +583,9 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
             uz[s] = (r[2] - c[2]) * iw;
         }
     }
+    std::vector<int32_t> slot_fleaf(n, -1);
+    for (size_t f = 0; f < fl_begin.size(); ++f)
+        for (int64_t sl = fl_begin[f]; sl < fl_end[f]; ++sl) slot_fleaf[sl] = static_cast<int32_t>(f);
     // ---- self lists (fmm.hpp:398-408): per target, listed sources other than
     // itself, flagged when in its near field (is_neighbor, partition.hpp:484-488)
     std::vector<int64_t> sc_tgt, sc_row{0}, sc_src;
@@ -715,6 +718,7 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
         (rc = P->upload(sc_src, &P->self.src)) || (rc = P->upload(sc_near, &P->self.near)))
         return rc;
     UP(fl_gnode, fl_gnode); UP(fl_level, fl_level); UP(fl_begin, fl_begin); UP(fl_end, fl_end);
+    UP(slot_fleaf, slot_fleaf);
     UP(lf_begin, lf_begin); UP(lf_end, lf_end);
     UP(t.near_row, near_row); UP(t.near_begin, near_begin); UP(t.near_end, near_end); UP(near_pref, near_pref);
     UP(octant, octant); UP(parent, parent); UP(child0, child0); UP(nchild, nchild);

@@ -68,15 +68,16 @@ __global__ void fill_kernel(double* x, int64_t count, double v)
 //   (dx + i dy) R(n,0) =  sqrt(n(n-1)) conj R(n-1,1)
 //   (dx - i dy) R(n,m) =  sqrt((n-m)(n-m-1)) R(n-1,m+1)
 //    dz         R(n,m) =  sqrt((n-m)(n+m))   R(n-1,m)
-template <int P>
-__global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int ncq, int nc, const double* __restrict__ q,
+template <int P, int NCQ>
+__global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int nc, const double* __restrict__ q,
                                                  double* __restrict__ M)
 {
     constexpr int K = (P + 1) * (P + 1);
     constexpr int NT = (P + 1) * (P + 2) / 2;  // complex coefficients with m >= 0
     constexpr int G = 64 / (P + 1);
-    __shared__ double part[G][4][2 * NT];
-    __shared__ double S[4][2 * NT];
+    constexpr int ncq = NCQ;
+    __shared__ double part[G][NCQ][2 * NT];
+    __shared__ double S[NCQ][2 * NT];
 
     const int64_t f = blockIdx.x;
     const int tid = threadIdx.x;
@@ -86,6 +87,13 @@ __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int ncq, i
     const int m = tid % (P + 1);
     const int g = tid / (P + 1);
     const double* rec = d.rec;
+    // this lane's recurrence coefficients a(n, m), b(n, m), loaded once
+    double ra[P + 1], rb[P + 1];
+#pragma unroll
+    for (int n = 0; n <= P; ++n) {
+        ra[n] = n > m ? rec_a(rec, n, m) : 0.0;
+        rb[n] = n > m ? rec_b(rec, n, m) : 0.0;
+    }
 
     for (int cq = 0; cq < ncq; ++cq) {
         double ar[P + 1], ai[P + 1];
@@ -116,7 +124,7 @@ __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int ncq, i
                         ar[n] += qi * dr;
                         ai[n] += qi * di;
                     } else if (n > m) {
-                        const double a = rec_a(rec, n, m) * z, bb = rec_b(rec, n, m) * r2;
+                        const double a = ra[n] * z, bb = rb[n] * r2;
                         const double rr = a * p1r - bb * p2r;
                         const double ri = a * p1i - bb * p2i;
                         p2r = p1r;
@@ -160,7 +168,7 @@ __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int ncq, i
             continue;
         }
         // ladder gradients of each moment component needed
-        double gx[3][2], gy[3][2], gz[3][2];
+        double gx[NCQ < 3 ? 3 : NCQ][2], gy[NCQ < 3 ? 3 : NCQ][2], gz[NCQ < 3 ? 3 : NCQ][2];
         for (int c = 0; c < ncq; ++c) {
             const double* Sc = S[c];
             double ppr = 0, ppi = 0, mmr = 0, mmi = 0, zr = 0, zi = 0;
@@ -219,56 +227,58 @@ __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int ncq, i
 }
 
 // ------------------------------------------------------------------ L2P
-// One wavefront per leaf, lane = particle: U_c += (1/w) sum_{n,m} Re(L~ conj R(n,m)),
-// the m > 0 terms doubled for the conjugate half (fmm.hpp:747-771).
+// One thread per particle of the far-field leaves (slot_fleaf maps a sorted
+// slot to its leaf): U_c += (1/w) sum_{n,m} Re(L~ conj R(n,m)), the m > 0 terms
+// doubled for the conjugate half (fmm.hpp:747-771).  The leaf's coefficients
+// are read through L1 (the lanes of a wavefront share a few leaves).  FP64 on
+// both precision paths (FP32 harmonics here cost 3x in potential error for a
+// few percent of the stage).
 template <int P>
-__global__ __launch_bounds__(64) void l2p_kernel(DevPlan d, int nc, const double* __restrict__ L,
-                                                 double* __restrict__ out)
+__global__ __launch_bounds__(256) void l2p_kernel(DevPlan d, int nc, const double* __restrict__ L,
+                                                  double* __restrict__ out)
 {
+    using R = double;
     constexpr int K = (P + 1) * (P + 1);
-    __shared__ double Ls[4][K];
-    const int64_t f = blockIdx.x;
-    const int tid = threadIdx.x;
-    const int32_t gnode = d.fl_gnode[f];
-    const int64_t b = d.fl_begin[f], e = d.fl_end[f];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= d.n) return;
+    const int32_t f = d.slot_fleaf[i];
+    if (f < 0) return;  // near-field-only leaf (level < 2)
     const double invw = d.inv_width[d.fl_level[f]];
-    const double* Lg = L + static_cast<int64_t>(gnode) * nc * K;
-    for (int v = tid; v < nc * K; v += 64) Ls[v / K][v % K] = Lg[v];
-    __syncthreads();
+    const double* Lg = L + static_cast<int64_t>(d.fl_gnode[f]) * nc * K;
     const double* rec = d.rec;
-    for (int64_t i = b + tid; i < e; i += 64) {
-        const double x = d.ux[i], y = d.uy[i], z = d.uz[i];
-        const double r2 = x * x + y * y + z * z;
-        double U[4] = {0.0, 0.0, 0.0, 0.0};
-        double dr = 1.0, di = 0.0;
+    const R x = static_cast<R>(d.ux[i]), y = static_cast<R>(d.uy[i]), z = static_cast<R>(d.uz[i]);
+    const R r2 = x * x + y * y + z * z;
+    double U[4] = {0.0, 0.0, 0.0, 0.0};
+    R dr = 1, di = 0;
 #pragma unroll
-        for (int m = 0; m <= P; ++m) {
-            if (m > 0) {
-                const double c = rec_d(rec, m);
-                const double tr = c * (-x * dr - y * di);
-                const double ti = c * (y * dr - x * di);
-                dr = tr;
-                di = ti;
-            }
-            const double fm = (m == 0) ? 1.0 : 2.0;
-            double p1r = dr, p1i = di, p2r = 0.0, p2i = 0.0;
-            for (int c = 0; c < nc; ++c)
-                U[c] += fm * (Ls[c][rfc(m, m)] * p1r + (m > 0 ? Ls[c][rfs(m, m)] * p1i : 0.0));
-#pragma unroll
-            for (int n = m + 1; n <= P; ++n) {
-                const double a = rec_a(rec, n, m) * z, bb = rec_b(rec, n, m) * r2;
-                const double rr = a * p1r - bb * p2r;
-                const double ri = a * p1i - bb * p2i;
-                p2r = p1r;
-                p2i = p1i;
-                p1r = rr;
-                p1i = ri;
-                for (int c = 0; c < nc; ++c)
-                    U[c] += fm * (Ls[c][rfc(n, m)] * rr + (m > 0 ? Ls[c][rfs(n, m)] * ri : 0.0));
-            }
+    for (int m = 0; m <= P; ++m) {
+        if (m > 0) {
+            const R c = static_cast<R>(rec_d(rec, m));
+            const R tr = c * (-x * dr - y * di);
+            const R ti = c * (y * dr - x * di);
+            dr = tr;
+            di = ti;
         }
-        for (int c = 0; c < nc; ++c) out[c * d.n + i] += invw * U[c];
+        R u[4] = {0, 0, 0, 0};
+        R p1r = dr, p1i = di, p2r = 0, p2i = 0;
+        for (int c = 0; c < nc; ++c)
+            u[c] += static_cast<R>(Lg[c * K + rfc(m, m)]) * p1r + (m > 0 ? static_cast<R>(Lg[c * K + rfs(m, m)]) * p1i : R(0));
+#pragma unroll
+        for (int n = m + 1; n <= P; ++n) {
+            const R a = static_cast<R>(rec_a(rec, n, m)) * z, bb = static_cast<R>(rec_b(rec, n, m)) * r2;
+            const R rr = a * p1r - bb * p2r;
+            const R ri = a * p1i - bb * p2i;
+            p2r = p1r;
+            p2i = p1i;
+            p1r = rr;
+            p1i = ri;
+            for (int c = 0; c < nc; ++c)
+                u[c] += static_cast<R>(Lg[c * K + rfc(n, m)]) * rr + (m > 0 ? static_cast<R>(Lg[c * K + rfs(n, m)]) * ri : R(0));
+        }
+        const double fm = (m == 0) ? 1.0 : 2.0;
+        for (int c = 0; c < nc; ++c) U[c] += fm * static_cast<double>(u[c]);
     }
+    for (int c = 0; c < nc; ++c) out[c * d.n + i] += invw * U[c];
 }
 
 // ------------------------------------------------------------------ M2M / L2L
@@ -726,15 +736,20 @@ template <int P>
 struct P2MRun {
     static void run(const DevPlan& d, int mode, int ncq, int nc, const double* q, double* M, hipStream_t s)
     {
-        hipLaunchKernelGGL(p2m_kernel<P>, dim3(static_cast<unsigned>(d.n_far_leaf)), dim3(64), 0, s, d, mode, ncq, nc,
-                           q, M);
+        const dim3 grid(static_cast<unsigned>(d.n_far_leaf)), blk(64);
+        switch (ncq) {
+        case 1: hipLaunchKernelGGL((p2m_kernel<P, 1>), grid, blk, 0, s, d, mode, nc, q, M); break;
+        case 2: hipLaunchKernelGGL((p2m_kernel<P, 2>), grid, blk, 0, s, d, mode, nc, q, M); break;
+        case 3: hipLaunchKernelGGL((p2m_kernel<P, 3>), grid, blk, 0, s, d, mode, nc, q, M); break;
+        default: hipLaunchKernelGGL((p2m_kernel<P, 4>), grid, blk, 0, s, d, mode, nc, q, M); break;
+        }
     }
 };
 template <int P>
 struct L2PRun {
     static void run(const DevPlan& d, int nc, const double* L, double* out, hipStream_t s)
     {
-        hipLaunchKernelGGL(l2p_kernel<P>, dim3(static_cast<unsigned>(d.n_far_leaf)), dim3(64), 0, s, d, nc, L, out);
+        hipLaunchKernelGGL(l2p_kernel<P>, dim3(grid_for(d.n, 256)), dim3(256), 0, s, d, nc, L, out);
     }
 };
 template <int P>

@@ -28,6 +28,7 @@ struct DevPlan {
     int64_t n_far_leaf = 0;
     const int32_t *fl_gnode = nullptr, *fl_level = nullptr;
     const int64_t *fl_begin = nullptr, *fl_end = nullptr;
+    const int32_t* slot_fleaf = nullptr;  // sorted slot -> far leaf index, -1 for near-only leaves
     // all leaves, near field
     int64_t n_leaf = 0;
     const int64_t *lf_begin = nullptr, *lf_end = nullptr;
