@@ -68,7 +68,7 @@ def cpu_baseline(gen: str, p: int, cap: int, sample_n: int, seed: int):
     mpiexec = "/opt/conda/bin/mpiexec"
     if not (os.path.exists(exe) and os.path.exists(mpiexec)):
         return None, "reference CPU build not present"
-    cores = os.cpu_count() or 1
+    cores = len(os.sched_getaffinity(0)) or 1  # the CPUs this process may run on
     pos, q = getattr(W, gen)(sample_n, seed=seed)
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "in.bin")
@@ -88,6 +88,21 @@ def cpu_baseline(gen: str, p: int, cap: int, sample_n: int, seed: int):
                    f"phi by rinv + grad phi by two rinv<true> (curl identity), {cores} MPI ranks; "
                    f"eval {t_eval:.2f}s, plan build {rec['ctor_s']:.2f}s, tree {rec['tree_s']:.2f}s"),
     }, None
+
+
+def traffic_from_profile(workload: str):
+    """HBM bytes per M2L launch {"fp64": B, "mixed": B} from the committed counter
+    summary of this workload (profiles/traffic_<workload>.md, line
+    `m2l_traffic_bytes: fp64 <B> mixed <B>`), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("m2l_traffic_bytes:"):
+                tok = line.split()
+                return {tok[i]: float(tok[i + 1]) for i in range(1, len(tok) - 1, 2)}
+    return None
 
 
 def main() -> int:
@@ -206,6 +221,9 @@ def main() -> int:
 
     paths = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     res = {pr: run_path(pr) for pr in paths}
+    traffic = traffic_from_profile(args.workload) if not args.n else None
+    for pr in paths:
+        res[pr]["roofline"]["traffic"] = traffic.get(pr) if traffic else None
     head = res["mixed"] if "mixed" in res else res["fp64"]
     line = {
         "metric": "points/sec (pot+grad) at rel err <=1e-6: mixed vs FP64 GPU, 1 MI355X",
