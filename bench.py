@@ -32,8 +32,10 @@ PEAK_FP32 = 157.3e12   # MI355X dense FP32 MFMA, FLOP/s
 PEAK_HBM = 8.0e12      # bytes/s
 
 WORKLOADS = {
-    # name: (generator, n, seed, p, cap)
+    # name: (generator, n, seed, p, cap); batch256: 256 independent cluster problems,
+    # sizes log-uniform in [1e4, 1e6], p from {6, 8, 10, 12} (n, seed, p unused)
     "plummer16m_p12": ("plummer", 1 << 24, 3, 12, 64),
+    "batch256": ("clusters", 0, 11, 0, 64),
     "uniform4m_p10": ("uniform_cube", 1 << 22, 1, 10, 64),
     "uniform100k_p8": ("uniform_cube", 100_000, 1, 8, 64),
 }
@@ -105,6 +107,89 @@ def traffic_from_profile(workload: str):
     return None
 
 
+def run_batch(args) -> int:
+    """Batched workload: every problem of the batch evaluated by one batch launch
+    per step (sfmm_batch_*: tuned number of concurrent streams), against the same
+    evaluations issued one plan at a time on the plans' own streams."""
+    import torch
+
+    if not torch.cuda.is_available():
+        log("[bench] no GPU visible")
+        return 3
+    specs = W.batch_specs(256)
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    probs = []
+    for n, p, seed in specs:
+        pos, q = W.clusters(n, seed=seed)
+        part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, 64)
+        probs.append(dict(n=n, p=p, pos=pos, q=q, part=part, dq=torch.from_numpy(q).to(dev),
+                          du=torch.zeros((n, 4), dtype=torch.float64, device=dev)))
+    total = sum(pb["n"] for pb in probs)
+    log(f"[bench] batch: {len(probs)} problems, {total} points, trees {time.time() - t0:.1f}s")
+    res = {}
+    paths = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    for prec in paths:
+        t_a = time.time()
+        fmms = [api.FMM3D(pb["part"], pb["p"], ncharge=1, precision=api.MIXED if prec == "mixed" else api.FP64)
+                for pb in probs]
+        items = [(f, api.POTGRAD, 1, pb["dq"].data_ptr(), pb["du"].data_ptr()) for f, pb in zip(fmms, probs)]
+        batch = api.Batch(items, streams=0)
+        t_b = time.time()
+        for _ in range(args.warmup):
+            batch.launch()
+        batch.synchronize()
+        ta = time.perf_counter()
+        for _ in range(args.steps):
+            batch.launch()
+        batch.synchronize()
+        ms = (time.perf_counter() - ta) * 1e3 / args.steps
+        # the same evaluations issued one plan at a time
+        ta = time.perf_counter()
+        for _ in range(args.steps):
+            for f, pb in zip(fmms, probs):
+                f.evaluate_device(api.POTGRAD, 1, pb["dq"].data_ptr(), pb["du"].data_ptr())
+            for f in fmms:
+                f.synchronize()
+        ms_eager = (time.perf_counter() - ta) * 1e3 / args.steps
+        # accuracy on the largest problem against the FP64 direct sum
+        big = max(range(len(probs)), key=lambda k: probs[k]["n"])
+        pb = probs[big]
+        pb["du"].zero_()
+        torch.cuda.synchronize()
+        fmms[big].evaluate_device(api.POTGRAD, 1, pb["dq"].data_ptr(), pb["du"].data_ptr())
+        fmms[big].synchronize()
+        tg = np.sort(np.random.default_rng(7).choice(pb["n"], min(args.targets, pb["n"]), replace=False))
+        ref = fmms[big].direct(api.POTGRAD, pb["q"], tg)
+        U = pb["du"].cpu().numpy()
+
+        def rel(a, b):
+            return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+        res[prec] = {"points_per_sec": total / (ms * 1e-3), "ms_per_step": ms, "streams": batch.streams,
+                     "eager_ms_per_step": ms_eager, "plan_build_s": t_b - t_a,
+                     "error_largest": {"n": pb["n"], "p": pb["p"], "targets": int(len(tg)),
+                                       "rel_l2_phi": rel(U[tg, 0], ref[:, 0]),
+                                       "rel_l2_grad": rel(U[tg, 1:], ref[:, 1:])}}
+        log(f"[bench] batch {prec}: {ms:.2f} ms/step batched ({batch.streams} streams), "
+            f"{ms_eager:.2f} ms eager; largest problem error {res[prec]['error_largest']}")
+        del batch, fmms
+    head = res["mixed"] if "mixed" in res else res["fp64"]
+    print(json.dumps({
+        "metric": "points/sec (pot+grad), batch of independent problems, 1 MI355X",
+        "value": head["points_per_sec"], "unit": "points/sec", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if "mixed" not in res else "f64+f32", "data": "synthetic",
+        "config": {"workload": "batch256", "problems": len(probs), "points": total, "generator": "clusters",
+                   "orders": [6, 8, 10, 12], "leaf_capacity": 64, "output": "phi+grad",
+                   "parallelism": "single GPU"},
+        "paths": res,
+        "speedup_mixed_vs_fp64": (res["mixed"]["points_per_sec"] / res["fp64"]["points_per_sec"]
+                                  if len(res) == 2 else None),
+    }), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +209,8 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
+    if args.workload == "batch256":
+        return run_batch(args)
     gen, n, seed, p, cap = WORKLOADS[args.workload]
     if args.n:
         n = args.n

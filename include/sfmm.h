@@ -134,6 +134,34 @@ int sfmm_recorded_stage_ms(sfmm_plan plan, float* ms, int capacity, int* recorde
 int sfmm_direct(sfmm_plan plan, int kind, int ncharge, const double* Q, int64_t ntarget, const int64_t* targets,
                 double* out);
 
+/* ---- Batches of independent problems.
+ * No counterpart in the reference (one problem per FMM3D object): many plans
+ * (each its own tree, order and charges) evaluated by one call, spread over
+ * concurrent streams. */
+typedef struct sfmm_batch_s* sfmm_batch;
+typedef struct {
+    sfmm_plan plan;     /* distinct plans, one device */
+    int kind;           /* SFMM_POTENTIAL, SFMM_CURL or SFMM_POTGRAD */
+    int ncharge;
+    int which;          /* as for sfmm_evaluate */
+    const double* dQ;   /* device charges, AoS original order */
+    double* dU;         /* device outputs, accumulated */
+} sfmm_batch_item;
+
+/* Prepares every plan for its item and assigns the items to concurrent streams
+ * (largest first).  streams: 1..8, or 0 to time evaluations with 1, 2, 4, 8
+ * streams (into scratch outputs) and keep the fastest.  The items' arrays must
+ * stay valid while the batch is used. */
+int sfmm_batch_create(const sfmm_batch_item* items, int count, int streams, sfmm_batch* out);
+/* Evaluates every item after the work queued on `stream` (hipStream_t; NULL: the
+ * batch's own stream), which then waits for all of them; asynchronous. */
+int sfmm_batch_launch(sfmm_batch batch, void* stream);
+/* Waits for launches on the batch's own stream. */
+int sfmm_batch_synchronize(sfmm_batch batch);
+/* The number of concurrent streams of the batch. */
+int sfmm_batch_streams(sfmm_batch batch, int* streams);
+int sfmm_batch_destroy(sfmm_batch batch);
+
 /* Text of the last error on this thread. */
 const char* sfmm_last_error(void);
 

@@ -57,6 +57,11 @@ class PlanDesc(ctypes.Structure):
                 ("self_row", _ip), ("self_val", _ip), ("m2l_kernel", ctypes.c_int)]
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [("plan", ctypes.c_void_p), ("kind", ctypes.c_int), ("ncharge", ctypes.c_int), ("which", ctypes.c_int),
+                ("dQ", ctypes.c_void_p), ("dU", ctypes.c_void_p)]
+
+
 # symbol -> (restype, argtypes); every function declared in include/sfmm.h
 SIGNATURES = {
     "sfmm_box_from_positions": (ctypes.c_int, [ctypes.c_int64, _dp, ctypes.c_int, ctypes.c_double, _dp]),
@@ -77,6 +82,12 @@ SIGNATURES = {
     "sfmm_recorded_stage_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_int)]),
     "sfmm_direct": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int64, _ip, _dp]),
+    "sfmm_batch_create": (ctypes.c_int, [ctypes.POINTER(BatchItem), ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "sfmm_batch_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "sfmm_batch_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfmm_batch_streams": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "sfmm_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sfmm_last_error": (ctypes.c_char_p, []),
 }
 
@@ -301,3 +312,41 @@ class FMM3D:
         out = np.zeros((len(targets), nout))
         _check(lib().sfmm_direct(self.h, kind, ncharge, _d(Q), len(targets), _i(targets), _d(out)))
         return out
+
+
+class Batch:
+    """Independent problems evaluated together over concurrent streams (sfmm_batch_*).
+
+    items: sequence of (fmm, kind, ncharge, q_ptr, u_ptr[, which]) with device
+    pointers (e.g. torch tensors' data_ptr()); the plans must be distinct and
+    the arrays must outlive the batch.  streams=0 tunes the stream count.
+    """
+
+    def __init__(self, items, streams: int = 0):
+        arr = (BatchItem * len(items))()
+        self._plans = []
+        for k, it in enumerate(items):
+            fmm, kind, ncharge, qp, up = it[:5]
+            which = it[5] if len(it) > 5 else BOTH
+            arr[k] = BatchItem(fmm.h, kind, ncharge, which, ctypes.c_void_p(qp), ctypes.c_void_p(up))
+            self._plans.append(fmm)  # keep the plans alive
+        h = ctypes.c_void_p()
+        _check(lib().sfmm_batch_create(arr, len(items), int(streams), ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def streams(self) -> int:
+        v = ctypes.c_int()
+        _check(lib().sfmm_batch_streams(self.h, ctypes.byref(v)))
+        return v.value
+
+    def launch(self, stream_ptr: int = 0):
+        _check(lib().sfmm_batch_launch(self.h, ctypes.c_void_p(stream_ptr) if stream_ptr else None))
+
+    def synchronize(self):
+        _check(lib().sfmm_batch_synchronize(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.sfmm_batch_destroy(self.h)
+            self.h = None

@@ -113,6 +113,23 @@ struct sfmm_plan_s {
     }
 };
 
+// A batch: the evaluations of several plans, spread over concurrent streams.
+struct sfmm_batch_s {
+    int device = 0;
+    std::vector<sfmm_batch_item> items;
+    std::vector<int> where;            // stream of each item
+    std::vector<hipStream_t> streams;  // [0] is the batch's own stream
+    std::vector<hipEvent_t> fork, join;
+    int nstream = 1;
+
+    ~sfmm_batch_s()
+    {
+        for (auto e : fork) (void)hipEventDestroy(e);
+        for (auto e : join) (void)hipEventDestroy(e);
+        for (auto st : streams) (void)hipStreamDestroy(st);
+    }
+};
+
 namespace {
 
 int mode_of(int kind, int ncharge, sfmm::Mode& mode, int& qdim, int& nc)
@@ -338,10 +355,10 @@ sfmm::LevelSpan level_blocks(const std::vector<int64_t>& first, int l)
     return {first[l], first[l + 1] - first[l]};
 }
 
-int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int which)
+// Checks the arguments of an evaluation and makes every buffer and work list it
+// needs (all allocation and upload happens here, none in run_on).
+int prepare(sfmm_plan_s* P, int kind, int ncharge, int which, sfmm::Mode& mode, int& qdim, int& nc)
 {
-    sfmm::Mode mode;
-    int qdim, nc;
     if (int rc = mode_of(kind, ncharge, mode, qdim, nc)) return rc;
     if ((which & ~(SFMM_BOTH | SFMM_SELF_LIST)) != 0 || (which & SFMM_BOTH) == 0)
         return fail(SFMM_ERR_ARG, "which must be SFMM_FAR, SFMM_NEAR or SFMM_BOTH, optionally | SFMM_SELF_LIST");
@@ -353,18 +370,38 @@ int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int
         if (int rc = build_m2l_blocks(P, nc)) return rc;
         if (int rc = build_translate_blocks(P, nc)) return rc;
     }
+    return SFMM_OK;
+}
+
+int run_on(sfmm_plan_s* P, sfmm::Mode mode, int nc, int qdim, const double* dQ, double* dU, int which,
+           hipStream_t s, bool timed);
+
+int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int which)
+{
+    sfmm::Mode mode;
+    int qdim, nc;
+    if (int rc = prepare(P, kind, ncharge, which, mode, qdim, nc)) return rc;
+    return run_on(P, mode, nc, qdim, dQ, dU, which, P->stream, true);
+}
+
+// The evaluation's launches on stream s; `timed` records the stage events
+// (off while a graph is being captured).
+int run_on(sfmm_plan_s* P, sfmm::Mode mode, int nc, int qdim, const double* dQ, double* dU, int which,
+           hipStream_t s, bool timed)
+{
     const sfmm::DevPlan& d = P->d;
-    hipStream_t s = P->stream;
     const int depth = P->tree->t.depth;
     hipEvent_t* ev = P->ev;
-    if (P->rec_used < P->rec_count) ev = &P->rec_ev[9 * static_cast<size_t>(P->rec_used++)];
-    HIP_TRY(hipEventRecord(ev[0], s));
+    if (timed && P->rec_used < P->rec_count) ev = &P->rec_ev[9 * static_cast<size_t>(P->rec_used++)];
+#define REC(k) \
+    if (timed) HIP_TRY(hipEventRecord(ev[k], s))
+    REC(0);
     sfmm::launch_gather_charges(d, qdim, dQ, P->qbuf, s);
     sfmm::launch_fill(P->obuf, static_cast<int64_t>(nc) * d.n, 0.0, s);
-    HIP_TRY(hipEventRecord(ev[1], s));
+    REC(1);
     const bool far = (which & SFMM_FAR) && depth >= 2;  // fmm.hpp:816-820
     if (far) sfmm::launch_p2m(d, mode, nc, P->qbuf, P->M, s);
-    HIP_TRY(hipEventRecord(ev[2], s));
+    REC(2);
     if (far)
         for (int l = depth - 1; l >= 2; --l) {
             if (P->m2l_kernel == 0)
@@ -372,7 +409,7 @@ int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int
             else
                 sfmm::launch_m2m(d, {P->level_first[l], P->level_first[l + 1] - P->level_first[l]}, nc, P->M, s);
         }
-    HIP_TRY(hipEventRecord(ev[3], s));
+    REC(3);
     if (far) {
         if (P->m2l_kernel == 0) {
             sfmm::M2LArgs& A = P->m2l[nc];
@@ -384,7 +421,7 @@ int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int
         }
         else sfmm::launch_m2l(d, nc, P->M, P->L, s);
     }
-    HIP_TRY(hipEventRecord(ev[4], s));
+    REC(4);
     if (far)
         for (int l = 3; l <= depth; ++l) {
             if (P->m2l_kernel == 0)
@@ -392,19 +429,65 @@ int run(sfmm_plan_s* P, int kind, int ncharge, const double* dQ, double* dU, int
             else
                 sfmm::launch_l2l(d, {P->level_first[l], P->level_first[l + 1] - P->level_first[l]}, nc, P->L, s);
         }
-    HIP_TRY(hipEventRecord(ev[5], s));
+    REC(5);
     if (far) sfmm::launch_l2p(d, nc, P->L, P->obuf, s);
-    HIP_TRY(hipEventRecord(ev[6], s));
+    REC(6);
     if (which & SFMM_NEAR) {
         sfmm::launch_p2p(d, P->p2p, mode, nc, P->qbuf, P->obuf, P->p2p_partial, s, P->precision == SFMM_MIXED);
         sfmm::launch_self_correction(d, P->self, mode, nc, (which & SFMM_SELF_LIST) != 0, P->qbuf, P->obuf, s);
     }
-    HIP_TRY(hipEventRecord(ev[7], s));
+    REC(7);
     sfmm::launch_scatter_outputs(d, nc, P->obuf, dU, s);
-    HIP_TRY(hipEventRecord(ev[8], s));
+    REC(8);
+#undef REC
     HIP_TRY(hipGetLastError());
-    P->last_ev = ev;
-    P->timed = true;
+    if (timed) {
+        P->last_ev = ev;
+        P->timed = true;
+    }
+    return SFMM_OK;
+}
+
+
+constexpr int kBatchMaxStreams = 8;
+
+// Items to streams, longest first (cost ~ particles x coefficients).
+std::vector<int> batch_assign(const std::vector<sfmm_batch_item>& items, int ns)
+{
+    std::vector<int> order(items.size()), out(items.size(), 0);
+    std::vector<double> cost(items.size()), load(ns, 0.0);
+    for (size_t i = 0; i < items.size(); ++i) {
+        const sfmm_plan_s* P = items[i].plan;
+        cost[i] = static_cast<double>(P->tree->t.n) * P->K;
+        order[i] = static_cast<int>(i);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    for (int i : order) {
+        const int k = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        out[i] = k;
+        load[k] += cost[i];
+    }
+    return out;
+}
+
+// Issues the items' evaluations on ns streams after the work already queued on
+// `origin`, and makes `origin` wait for all of them.
+int batch_issue(sfmm_batch_s* B, const std::vector<sfmm_batch_item>& items, const std::vector<int>& where, int ns,
+                hipStream_t origin)
+{
+    HIP_TRY(hipEventRecord(B->fork[0], origin));
+    for (int k = 0; k < ns; ++k) HIP_TRY(hipStreamWaitEvent(B->streams[k], B->fork[0], 0));
+    for (size_t i = 0; i < items.size(); ++i) {
+        const sfmm_batch_item& it = items[i];
+        sfmm::Mode mode;
+        int qdim, nc;
+        if (int r = mode_of(it.kind, it.ncharge, mode, qdim, nc)) return r;
+        if (int r = run_on(it.plan, mode, nc, qdim, it.dQ, it.dU, it.which, B->streams[where[i]], false)) return r;
+    }
+    for (int k = 0; k < ns; ++k) {
+        HIP_TRY(hipEventRecord(B->join[k], B->streams[k]));
+        HIP_TRY(hipStreamWaitEvent(origin, B->join[k], 0));
+    }
     return SFMM_OK;
 }
 
@@ -916,6 +999,122 @@ int sfmm_direct(sfmm_plan plan, int kind, int ncharge, const double* Q, int64_t 
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     cleanup();
     if (e != hipSuccess) return fail(SFMM_ERR_HIP, std::string("direct sum: ") + hipGetErrorString(e));
+    return SFMM_OK;
+}
+
+
+int sfmm_batch_create(const sfmm_batch_item* items, int count, int streams, sfmm_batch* out)
+{
+    if (!items || !out || count <= 0) return fail(SFMM_ERR_ARG, "null argument or empty batch");
+    *out = nullptr;
+    if (streams < 0 || streams > kBatchMaxStreams) return fail(SFMM_ERR_ARG, "streams must be in 0..8 (0: tuned)");
+    auto B = std::make_unique<sfmm_batch_s>();
+    B->items.assign(items, items + count);
+    std::vector<const sfmm_plan_s*> seen;
+    for (const auto& it : B->items) {
+        if (!it.plan || !it.dQ || !it.dU) return fail(SFMM_ERR_ARG, "batch item with a null plan or array");
+        if (std::find(seen.begin(), seen.end(), it.plan) != seen.end())
+            return fail(SFMM_ERR_ARG, "a plan appears twice in the batch (one plan is not reentrant)");
+        seen.push_back(it.plan);
+        if (it.plan->device != items[0].plan->device) return fail(SFMM_ERR_ARG, "batch plans on different devices");
+    }
+    B->device = items[0].plan->device;
+    HIP_TRY(hipSetDevice(B->device));
+    // every buffer and work list exists before the capture
+    std::vector<int64_t> nout(count);
+    for (int i = 0; i < count; ++i) {
+        const auto& it = B->items[i];
+        sfmm::Mode mode;
+        int qdim, nc;
+        if (int rc = prepare(it.plan, it.kind, it.ncharge, it.which, mode, qdim, nc)) return rc;
+        HIP_TRY(hipStreamSynchronize(it.plan->stream));
+        nout[i] = static_cast<int64_t>(nc) * it.plan->tree->t.n;
+    }
+    B->streams.resize(kBatchMaxStreams);
+    for (auto& st : B->streams) HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    B->fork.resize(1);
+    B->join.resize(kBatchMaxStreams);
+    for (auto& e : B->fork) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : B->join) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+
+    int ns = std::min(streams, count);
+    if (streams == 0) {
+        // tuning: time evaluations with 1, 2, 4, 8 streams into scratch outputs
+        std::vector<double*> scratch(count, nullptr);
+        auto release = [&]() {
+            for (double* p : scratch) (void)hipFree(p);
+        };
+        std::vector<sfmm_batch_item> trial = B->items;
+        for (int i = 0; i < count; ++i) {
+            if (hipMalloc(&scratch[i], sizeof(double) * nout[i]) != hipSuccess) {
+                release();
+                return fail(SFMM_ERR_ALLOC, "hipMalloc failed (batch tuning outputs)");
+            }
+            trial[i].dU = scratch[i];
+        }
+        hipEvent_t t0, t1;
+        HIP_TRY(hipEventCreate(&t0));
+        HIP_TRY(hipEventCreate(&t1));
+        hipStream_t s0 = B->streams[0];
+        float best = -1.0f;
+        int rc = SFMM_OK;
+        for (int cand = 1; cand <= std::min(count, kBatchMaxStreams) && rc == SFMM_OK; cand *= 2) {
+            const std::vector<int> where = batch_assign(trial, cand);
+            if ((rc = batch_issue(B.get(), trial, where, cand, s0)) != SFMM_OK) break;  // warm-up
+            float ms = 0.0f;
+            hipError_t e = hipEventRecord(t0, s0);
+            for (int r = 0; r < 2 && e == hipSuccess && rc == SFMM_OK; ++r) rc = batch_issue(B.get(), trial, where, cand, s0);
+            if (e == hipSuccess) e = hipEventRecord(t1, s0);
+            if (e == hipSuccess) e = hipEventSynchronize(t1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+            if (rc != SFMM_OK) break;
+            if (e != hipSuccess) rc = fail(SFMM_ERR_HIP, std::string("batch tuning: ") + hipGetErrorString(e));
+            else if (best < 0.0f || ms < best) {
+                best = ms;
+                ns = cand;
+            }
+        }
+        (void)hipStreamSynchronize(s0);
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        release();
+        if (rc != SFMM_OK) return rc;
+    }
+    B->nstream = ns;
+    B->where = batch_assign(B->items, ns);
+    *out = B.release();
+    return SFMM_OK;
+}
+
+int sfmm_batch_launch(sfmm_batch batch, void* stream)
+{
+    if (!batch) return fail(SFMM_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(batch->device));
+    return batch_issue(batch, batch->items, batch->where, batch->nstream,
+                       stream ? static_cast<hipStream_t>(stream) : batch->streams[0]);
+}
+
+int sfmm_batch_synchronize(sfmm_batch batch)
+{
+    if (!batch) return fail(SFMM_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(batch->device));
+    HIP_TRY(hipStreamSynchronize(batch->streams[0]));
+    return SFMM_OK;
+}
+
+int sfmm_batch_streams(sfmm_batch batch, int* streams)
+{
+    if (!batch || !streams) return fail(SFMM_ERR_ARG, "null argument");
+    *streams = batch->nstream;
+    return SFMM_OK;
+}
+
+int sfmm_batch_destroy(sfmm_batch batch)
+{
+    if (!batch) return SFMM_OK;
+    (void)hipSetDevice(batch->device);
+    (void)hipStreamSynchronize(batch->streams[0]);
+    delete batch;
     return SFMM_OK;
 }
 
