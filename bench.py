@@ -235,12 +235,20 @@ def main() -> int:
         log(f"[bench] copied {args.copy_bytes} bytes")
         del src, dst
     pos, q = getattr(W, gen)(n, seed=seed)
+    # the tree on the host (all CPU threads) and on the GPU: the same tree, array for array
     t0 = time.time()
-    part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, cap)
-    t1 = time.time()
+    host_part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, cap)
+    t_host = time.time() - t0
+    t0 = time.time()
+    part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, cap, on_device=True)
+    t_dev = time.time() - t0
+    same_tree = part.digest() == host_part.digest()
+    del host_part
     info = part.info()
-    log(f"[bench] tree {t1 - t0:.2f}s (depth {info.depth}, {info.n_node} nodes, {info.n_leaf} leaves, "
-        f"{info.m2l_pairs} M2L pairs, {info.near_pairs} near pairs)")
+    log(f"[bench] tree: host {t_host:.2f}s, device {t_dev:.2f}s, identical {same_tree} (depth {info.depth}, "
+        f"{info.n_node} nodes, {info.n_leaf} leaves, {info.m2l_pairs} M2L pairs, {info.near_pairs} near pairs)")
+    if not same_tree:
+        raise SystemExit("device tree differs from the host tree")
 
     dev = torch.device("cuda:0")
     dq = torch.from_numpy(q).to(dev)
@@ -335,7 +343,8 @@ def main() -> int:
         "speedup_mixed_vs_fp64": (res["mixed"]["points_per_sec"] / res["fp64"]["points_per_sec"]
                                   if len(res) == 2 else None),
         "tree": {"depth": info.depth, "nodes": info.n_node, "leaves": info.n_leaf, "m2l_pairs": info.m2l_pairs,
-                 "near_pairs": info.near_pairs, "tree_build_s": t1 - t0},
+                 "near_pairs": info.near_pairs, "tree_build_device_s": t_dev, "tree_build_host_s": t_host,
+                 "identical": same_tree},
         "cpu_baseline": cpu if cpu else {"value": None, "note": cpu_note},
     }
     print(json.dumps(line), flush=True)

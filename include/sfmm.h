@@ -60,6 +60,7 @@ typedef struct {
     double corner[3];
     double width;
     int threads;               /* host threads for the build (0 = hardware)           */
+    int on_device;             /* 1: build on the current GPU (same tree, bit for bit) */
 } sfmm_tree_desc;
 
 /* Box(positions, vertical_axis, phideg) alone (box.hpp:84-107): out4 = width, corner x, y, z. */
@@ -69,6 +70,11 @@ int sfmm_box_from_positions(int64_t n, const double* positions, int vertical_axi
  * box.hpp:73-107, partition.hpp:397-446.  The tree copies the positions. */
 int sfmm_tree_create(const sfmm_tree_desc* desc, sfmm_tree* out);
 int sfmm_tree_destroy(sfmm_tree tree);
+
+/* FNV-1a hash over every array of the tree (permutation, per-level codes, ranges,
+ * links, leaves, neighbour and interaction lists, near ranges): equal digests mean
+ * the same tree.  No counterpart in the reference; it compares host and device builds. */
+int sfmm_tree_digest(sfmm_tree tree, uint64_t* digest);
 
 typedef struct {
     int depth;             /* partitioner.level()            */

@@ -43,7 +43,7 @@ class TreeDesc(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("positions", _dp), ("max_level", ctypes.c_int),
                 ("max_per_leaf", ctypes.c_int), ("vertical_axis", ctypes.c_int), ("phideg", ctypes.c_double),
                 ("box_given", ctypes.c_int), ("corner", ctypes.c_double * 3), ("width", ctypes.c_double),
-                ("threads", ctypes.c_int)]
+                ("threads", ctypes.c_int), ("on_device", ctypes.c_int)]
 
 
 class TreeInfo(ctypes.Structure):
@@ -67,6 +67,7 @@ SIGNATURES = {
     "sfmm_box_from_positions": (ctypes.c_int, [ctypes.c_int64, _dp, ctypes.c_int, ctypes.c_double, _dp]),
     "sfmm_tree_create": (ctypes.c_int, [ctypes.POINTER(TreeDesc), ctypes.POINTER(ctypes.c_void_p)]),
     "sfmm_tree_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfmm_tree_digest": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
     "sfmm_tree_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TreeInfo)]),
     "sfmm_tree_level_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip]),
     "sfmm_tree_level_lists": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_ip] * 11),
@@ -153,7 +154,10 @@ class OctreePartitioner:
         self._box = box
         self.h = None
 
-    def refine(self, max_level: int = 20, max_particles_per_node: int = 128, threads: int = 0):
+    def refine(self, max_level: int = 20, max_particles_per_node: int = 128, threads: int = 0,
+               on_device: bool = False):
+        """Builds the tree with `threads` host threads, or on the current GPU when
+        `on_device` (the same tree, array for array)."""
         desc = TreeDesc()
         desc.n = self._pos.shape[0]
         desc.positions = _d(self._pos)
@@ -167,6 +171,7 @@ class OctreePartitioner:
                 desc.corner[d] = self._box.corner[d]
             desc.width = self._box.width_
         desc.threads = int(threads)
+        desc.on_device = int(bool(on_device))
         h = ctypes.c_void_p()
         _check(lib().sfmm_tree_create(ctypes.byref(desc), ctypes.byref(h)))
         self.h = h
@@ -188,6 +193,12 @@ class OctreePartitioner:
 
     def level(self) -> int:
         return self.info().depth
+
+    def digest(self) -> int:
+        """FNV-1a hash over every array of the tree; equal digests mean the same tree."""
+        d = ctypes.c_uint64()
+        _check(lib().sfmm_tree_digest(self.h, ctypes.byref(d)))
+        return int(d.value)
 
     def level_lists(self, l: int) -> dict:
         c = np.zeros(6, dtype=np.int64)

@@ -493,6 +493,24 @@ int batch_issue(sfmm_batch_s* B, const std::vector<sfmm_batch_item>& items, cons
 
 }  // namespace
 
+namespace {
+struct Fnv {
+    uint64_t h = 1469598103934665603ULL;
+    void bytes(const void* p, size_t n)
+    {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ULL;
+    }
+    template <class T>
+    void vec(const std::vector<T>& v)
+    {
+        const uint64_t n = v.size();
+        bytes(&n, sizeof n);
+        bytes(v.data(), v.size() * sizeof(T));
+    }
+};
+}  // namespace
+
 extern "C" {
 
 const char* sfmm_last_error(void) { return g_last_error.c_str(); }
@@ -520,7 +538,10 @@ int sfmm_tree_create(const sfmm_tree_desc* desc, sfmm_tree* out)
                             : sfmm::make_box(desc->n, T->pos.data(), desc->vertical_axis, desc->phideg);
     int threads = desc->threads > 0 ? desc->threads : static_cast<int>(std::thread::hardware_concurrency());
     if (threads <= 0) threads = 1;
-    std::string err = sfmm::build_tree(T->t, desc->n, T->pos.data(), box, desc->max_level, desc->max_per_leaf, threads);
+    std::string err =
+        desc->on_device
+            ? sfmm::build_tree_device(T->t, desc->n, T->pos.data(), box, desc->max_level, desc->max_per_leaf)
+            : sfmm::build_tree(T->t, desc->n, T->pos.data(), box, desc->max_level, desc->max_per_leaf, threads);
     if (!err.empty()) return fail(SFMM_ERR_ARG, err);
     *out = T.release();
     return SFMM_OK;
@@ -529,6 +550,25 @@ int sfmm_tree_create(const sfmm_tree_desc* desc, sfmm_tree* out)
 int sfmm_tree_destroy(sfmm_tree tree)
 {
     delete tree;
+    return SFMM_OK;
+}
+
+int sfmm_tree_digest(sfmm_tree tree, uint64_t* digest)
+{
+    if (!tree || !digest) return fail(SFMM_ERR_ARG, "null argument");
+    const sfmm::Tree& t = tree->t;
+    Fnv f;
+    const int64_t head[4] = {t.n, t.max_level, t.cap, t.depth};
+    f.bytes(head, sizeof head);
+    f.vec(t.perm);
+    for (const sfmm::TreeLevel& l : t.lv) {
+        f.vec(l.code), f.vec(l.pbegin), f.vec(l.pend), f.vec(l.parent), f.vec(l.child0), f.vec(l.nchild);
+        f.vec(l.node2leaf), f.vec(l.leaf2node), f.vec(l.nb_row), f.vec(l.nb), f.vec(l.il_row), f.vec(l.il_src);
+        f.vec(l.il_off);
+    }
+    f.vec(t.leaf_level), f.vec(t.leaf_node), f.vec(t.leaf_row), f.vec(t.near_row), f.vec(t.near_begin);
+    f.vec(t.near_end);
+    *digest = f.h;
     return SFMM_OK;
 }
 

@@ -2,6 +2,8 @@
 // kept from the reference partitioner and the construction used here.
 #include "tree.hpp"
 
+#include "morton.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <functional>
@@ -28,28 +30,6 @@ void parallel_chunks(int64_t n, int threads, const std::function<void(int64_t, i
     for (auto& th : pool) th.join();
 }
 
-inline int64_t spread3(uint32_t v)  // bit b -> bit 3b, 21 bits
-{
-    uint64_t x = v & 0x1fffff;
-    x = (x | x << 32) & 0x1f00000000ffffULL;
-    x = (x | x << 16) & 0x1f0000ff0000ffULL;
-    x = (x | x << 8) & 0x100f00f00f00f00fULL;
-    x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
-    x = (x | x << 2) & 0x1249249249249249ULL;
-    return static_cast<int64_t>(x);
-}
-
-inline uint32_t compact3(uint64_t x)  // inverse of spread3
-{
-    x &= 0x1249249249249249ULL;
-    x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ULL;
-    x = (x ^ (x >> 4)) & 0x100f00f00f00f00fULL;
-    x = (x ^ (x >> 8)) & 0x1f0000ff0000ffULL;
-    x = (x ^ (x >> 16)) & 0x1f00000000ffffULL;
-    x = (x ^ (x >> 32)) & 0x1fffffULL;
-    return static_cast<uint32_t>(x);
-}
-
 // Stable LSD radix sort of (key, value) by key, `bits` low bits significant.
 void radix_sort(std::vector<int64_t>& key, std::vector<int32_t>& val, int bits)
 {
@@ -71,11 +51,9 @@ void radix_sort(std::vector<int64_t>& key, std::vector<int32_t>& val, int bits)
     }
 }
 
-inline int offset_id(int di, int dj, int dk) { return ((dk + 3) * 7 + (dj + 3)) * 7 + (di + 3); }
-
 }  // namespace
 
-int32_t morton_axis(int64_t code, int axis) { return static_cast<int32_t>(compact3(static_cast<uint64_t>(code) >> axis)); }
+int32_t morton_axis(int64_t code, int axis) { return code_axis(code, axis); }
 
 int64_t morton_code(int32_t i, int32_t j, int32_t k)
 {
@@ -85,8 +63,7 @@ int64_t morton_code(int32_t i, int32_t j, int32_t k)
 
 void BoxGeom::rotate(const double* v, double* r) const  // box.hpp:152-157
 {
-    r[0] = cphi * v[X] - sphi * v[Y];
-    r[1] = sphi * v[X] + cphi * v[Y];
+    rotate_xy(cphi, sphi, v[X], v[Y], r[0], r[1]);
     r[2] = v[Z];
 }
 
@@ -174,21 +151,12 @@ std::string build_tree(Tree& t, int64_t n, const double* pos, const BoxGeom& box
     t.perm.resize(n);
     std::vector<char> bad(threads, 0);
     parallel_chunks(n, threads, [&](int64_t a, int64_t b, int c) {
-        const int32_t imax = (int32_t(1) << L) - 1;
         for (int64_t i = a; i < b; ++i) {
             double r[3];
             box.rotate(pos + 3 * i, r);
             int32_t iv[3];
-            for (int d = 0; d < 3; ++d) {
-                const double u = (r[d] - box.corner[d]) / box.width;
-                // box.hpp:60 asserts 0 <= u <= 1
-                if (!(u >= 0 && u <= 1)) {
-                    bad[c] = 1;
-                    iv[d] = 0;
-                    continue;
-                }
-                iv[d] = std::min(std::max(static_cast<int32_t>(u * (1 << L)), int32_t(0)), imax);
-            }
+            for (int d = 0; d < 3; ++d)
+                if (!cell_index(r[d], box.corner[d], box.width, L, iv[d])) bad[c] = 1;
             key[i] = morton_code(iv[0], iv[1], iv[2]);
             t.perm[i] = static_cast<int32_t>(i);
         }

@@ -81,6 +81,10 @@ struct Tree {
 std::string build_tree(Tree& t, int64_t n, const double* pos, const BoxGeom& box, int max_level, int cap,
                        int threads);
 
+// The same tree built on the current GPU (codes, sort, levels, leaves, lists and
+// near ranges in HBM) and copied into `t`; every array equals build_tree's.
+std::string build_tree_device(Tree& t, int64_t n, const double* pos, const BoxGeom& box, int max_level, int cap);
+
 // Reference-format lists of one level (partition.hpp:155-166), for parity checks.
 struct RefLists {
     std::vector<int64_t> c_node, leaf2node, idx_row, idx_val, ilist_row, ilist_val, clist_row, clist_val, plist,
