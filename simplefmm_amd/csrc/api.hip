@@ -12,9 +12,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <sched.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <thread>
@@ -23,6 +26,30 @@
 namespace {
 
 thread_local std::string g_last_error;
+
+// Host threads for plan construction: the CPUs this process may run on.
+int host_threads()
+{
+    cpu_set_t set;
+    int n = 0;
+    if (sched_getaffinity(0, sizeof set, &set) == 0) n = CPU_COUNT(&set);
+    if (n <= 0) n = static_cast<int>(std::thread::hardware_concurrency());
+    return std::max(1, n);
+}
+
+// f(begin, end, chunk) over [0, count) cut into `chunks` contiguous ranges, one thread each.
+void parallel_ranges(int64_t count, int chunks, const std::function<void(int64_t, int64_t, int)>& f)
+{
+    chunks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(chunks, count / 1024)));
+    if (chunks <= 1) {
+        f(0, count, 0);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int c = 0; c < chunks; ++c)
+        pool.emplace_back([&f, count, chunks, c] { f(count * c / chunks, count * (c + 1) / chunks, c); });
+    for (auto& t : pool) t.join();
+}
 
 int fail(int code, const std::string& msg)
 {
@@ -202,10 +229,9 @@ int build_m2l_blocks(sfmm_plan_s* P, int nc)
     if (A.blk_row) return SFMM_OK;
     const sfmm::Tree& t = P->tree->t;
     const int TB = 16 / nc;
-    std::vector<int32_t> blk_tgt, ent_src, src(343 * TB);
+    std::vector<int32_t> blk_tgt, ent_src;
     std::vector<int64_t> blk_row{0};
     std::vector<int32_t> ent_meta;
-    std::vector<char> used(343, 0);
     // offsets in (polar class, distance class, id) order: consecutive offsets
     // share the rotation operators, and mostly the coaxial one as well
     const auto& cls = P->tables.off_class;
@@ -216,43 +242,68 @@ int build_m2l_blocks(sfmm_plan_s* P, int nc)
     std::sort(order.begin(), order.end(), [&](int x, int y) {
         return cls[x] != cls[y] ? cls[x] < cls[y] : d2[x] != d2[y] ? d2[x] < d2[y] : x < y;
     });
-    std::vector<int32_t> oct_nodes[8];
+    // blocks in launch order: per level, the q-th chunks of the 8 octants
+    // (children of nearby parents) are adjacent
+    struct Blk {
+        int level, octant;
+        int64_t first;  // index of the first target in the level's octant list
+    };
+    std::vector<std::vector<int32_t>> oct_nodes(static_cast<size_t>(8) * (t.depth + 1));
+    std::vector<Blk> blocks;
     for (int l = 2; l <= t.depth; ++l) {
         const auto& lv = t.lv[l];
-        for (auto& v : oct_nodes) v.clear();
-        for (int32_t a = 0; a < lv.n_node(); ++a) oct_nodes[lv.code[a] & 7].push_back(a);
+        for (int32_t a = 0; a < lv.n_node(); ++a) oct_nodes[8 * l + (lv.code[a] & 7)].push_back(a);
         size_t nchunk = 0;
-        for (const auto& v : oct_nodes) nchunk = std::max(nchunk, (v.size() + TB - 1) / TB);
-        // the q-th chunks of the 8 octants (children of nearby parents) are adjacent
+        for (int o = 0; o < 8; ++o) nchunk = std::max(nchunk, (oct_nodes[8 * l + o].size() + TB - 1) / TB);
         for (size_t q = 0; q < nchunk; ++q)
-            for (int o = 0; o < 8; ++o) {
-                const auto& nodes = oct_nodes[o];
-                const size_t b = q * TB;
-                if (b >= nodes.size()) continue;
-                std::fill(src.begin(), src.end(), -1);
-                for (int tt = 0; tt < TB; ++tt) {
-                    if (b + tt >= nodes.size()) {
-                        blk_tgt.push_back(-1);
-                        continue;
-                    }
-                    const int32_t a = nodes[b + tt];
-                    blk_tgt.push_back(static_cast<int32_t>(P->level_first[l] + a));
-                    for (int64_t k = lv.il_row[a]; k < lv.il_row[a + 1]; ++k) {
-                        const int off = lv.il_off[k];
-                        src[off * TB + tt] = static_cast<int32_t>(P->level_first[l] + lv.il_src[k]);
-                        used[off] = 1;
-                    }
+            for (int o = 0; o < 8; ++o)
+                if (q * TB < oct_nodes[8 * l + o].size()) blocks.push_back({l, o, static_cast<int64_t>(q * TB)});
+    }
+    // each thread fills a contiguous range of blocks; the ranges are concatenated in order
+    const int nth = host_threads();
+    std::vector<std::vector<int32_t>> t_tgt(nth), t_meta(nth), t_src(nth);
+    std::vector<std::vector<int64_t>> t_cnt(nth);
+    std::vector<char> t_bad(nth, 0);
+    parallel_ranges(static_cast<int64_t>(blocks.size()), nth, [&](int64_t b0, int64_t b1, int c) {
+        std::vector<int32_t> src(343 * TB);
+        std::vector<char> used(343, 0);
+        for (int64_t bi = b0; bi < b1; ++bi) {
+            const Blk& B = blocks[bi];
+            const auto& lv = t.lv[B.level];
+            const auto& nodes = oct_nodes[8 * B.level + B.octant];
+            std::fill(src.begin(), src.end(), -1);
+            for (int tt = 0; tt < TB; ++tt) {
+                if (B.first + tt >= static_cast<int64_t>(nodes.size())) {
+                    t_tgt[c].push_back(-1);
+                    continue;
                 }
-                for (int off : order) {
-                    if (!used[off]) continue;
-                    used[off] = 0;
-                    ent_meta.push_back(off | cls[off] << 9 | d2[off] << 16);
-                    ent_src.insert(ent_src.end(), src.begin() + off * TB, src.begin() + (off + 1) * TB);
+                const int32_t a = nodes[B.first + tt];
+                t_tgt[c].push_back(static_cast<int32_t>(P->level_first[B.level] + a));
+                for (int64_t k = lv.il_row[a]; k < lv.il_row[a + 1]; ++k) {
+                    const int off = lv.il_off[k];
+                    src[off * TB + tt] = static_cast<int32_t>(P->level_first[B.level] + lv.il_src[k]);
+                    used[off] = 1;
                 }
-                blk_row.push_back(static_cast<int64_t>(ent_meta.size()));
-                if (blk_row.back() - blk_row[blk_row.size() - 2] > 189)
-                    return fail(SFMM_ERR_STATE, "M2L block with more than 189 interaction offsets");
             }
+            int64_t cnt = 0;
+            for (int off : order) {
+                if (!used[off]) continue;
+                used[off] = 0;
+                t_meta[c].push_back(off | cls[off] << 9 | d2[off] << 16);
+                t_src[c].insert(t_src[c].end(), src.begin() + off * TB, src.begin() + (off + 1) * TB);
+                ++cnt;
+            }
+            if (cnt > 189) t_bad[c] = 1;
+            t_cnt[c].push_back(cnt);
+        }
+    });
+    for (char bad : t_bad)
+        if (bad) return fail(SFMM_ERR_STATE, "M2L block with more than 189 interaction offsets");
+    for (int c = 0; c < nth; ++c) {
+        blk_tgt.insert(blk_tgt.end(), t_tgt[c].begin(), t_tgt[c].end());
+        ent_meta.insert(ent_meta.end(), t_meta[c].begin(), t_meta[c].end());
+        ent_src.insert(ent_src.end(), t_src[c].begin(), t_src[c].end());
+        for (int64_t v : t_cnt[c]) blk_row.push_back(blk_row.back() + v);
     }
     A.n_block = static_cast<int64_t>(blk_row.size()) - 1;
     int rc;
@@ -675,13 +726,16 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
     const int64_t n = t.n;
     std::vector<double> px(n), py(n), pz(n), ux(n, 0.0), uy(n, 0.0), uz(n, 0.0);
     std::vector<int32_t> iperm(n);
-    for (int64_t s = 0; s < n; ++s) {
-        const double* x = tree->pos.data() + 3 * static_cast<int64_t>(t.perm[s]);
-        px[s] = x[0];
-        py[s] = x[1];
-        pz[s] = x[2];
-        iperm[t.perm[s]] = static_cast<int32_t>(s);
-    }
+    const int nthreads = host_threads();
+    parallel_ranges(n, nthreads, [&](int64_t s0, int64_t s1, int) {
+        for (int64_t s = s0; s < s1; ++s) {
+            const double* x = tree->pos.data() + 3 * static_cast<int64_t>(t.perm[s]);
+            px[s] = x[0];
+            py[s] = x[1];
+            pz[s] = x[2];
+            iperm[t.perm[s]] = static_cast<int32_t>(s);
+        }
+    });
     std::vector<int32_t> fl_gnode, fl_level;
     std::vector<int64_t> fl_begin, fl_end, lf_begin, lf_end;
     for (size_t g = 0; g < t.leaf_node.size(); ++g) {
@@ -695,17 +749,24 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
         fl_level.push_back(l);
         fl_begin.push_back(lv.pbegin[a]);
         fl_end.push_back(lv.pend[a]);
-        double c[3];
-        t.box.centre(l, lv.code[a], c);
-        const double iw = std::ldexp(1.0, l) / t.box.width;
-        for (int64_t s = lv.pbegin[a]; s < lv.pend[a]; ++s) {
-            double r[3];
-            t.box.rotate(tree->pos.data() + 3 * static_cast<int64_t>(t.perm[s]), r);
-            ux[s] = (r[0] - c[0]) * iw;
-            uy[s] = (r[1] - c[1]) * iw;
-            uz[s] = (r[2] - c[2]) * iw;
-        }
     }
+    // particle coordinates relative to their far leaf's centre, in leaf widths
+    parallel_ranges(static_cast<int64_t>(fl_gnode.size()), nthreads, [&](int64_t f0, int64_t f1, int) {
+        for (int64_t f = f0; f < f1; ++f) {
+            const int l = fl_level[f];
+            const int32_t a = static_cast<int32_t>(fl_gnode[f] - P->level_first[l]);
+            double c[3];
+            t.box.centre(l, t.lv[l].code[a], c);
+            const double iw = std::ldexp(1.0, l) / t.box.width;
+            for (int64_t s = fl_begin[f]; s < fl_end[f]; ++s) {
+                double r[3];
+                t.box.rotate(tree->pos.data() + 3 * static_cast<int64_t>(t.perm[s]), r);
+                ux[s] = (r[0] - c[0]) * iw;
+                uy[s] = (r[1] - c[1]) * iw;
+                uz[s] = (r[2] - c[2]) * iw;
+            }
+        }
+    });
     std::vector<int32_t> slot_fleaf(n, -1);
     for (size_t f = 0; f < fl_begin.size(); ++f)
         for (int64_t sl = fl_begin[f]; sl < fl_end[f]; ++sl) slot_fleaf[sl] = static_cast<int32_t>(f);
@@ -743,11 +804,12 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
             }
         }
     }
-    // ---- M2L work: all nodes at levels >= 2
+    // ---- M2L work of the row kernel: all nodes at levels >= 2 (the MFMA kernel
+    //      builds its own offset-grouped blocks from the tree's lists)
     std::vector<int32_t> tgt, il_src;
     std::vector<int16_t> il_off;
     std::vector<int64_t> il_row{0};
-    for (int l = 2; l <= D; ++l) {
+    for (int l = 2; l <= D && P->m2l_kernel == 1; ++l) {
         const auto& lv = t.lv[l];
         for (int32_t a = 0; a < lv.n_node(); ++a) {
             tgt.push_back(static_cast<int32_t>(P->level_first[l] + a));
