@@ -5,7 +5,8 @@
 // The O(N^2) check of main.cpp:70-96 runs on the GPU over a sample of targets
 // (sfmm_direct) instead of all N.
 //
-// usage: sfmm_driver [N] [max_particles_per_node] [n_targets] [seed] [--in FILE] [--out FILE]
+// usage: sfmm_driver [N] [max_particles_per_node] [n_targets] [seed] [--in FILE] [--out FILE] [--device-tree]
+//   --device-tree: build the octree on the GPU (the same tree as the host build)
 //   --in  FILE: 'SFMI' | int32 1 | int64 n | int32 qdim (1 or 3) | double pos[3n] | double q[qdim n]
 //   --out FILE: 'SFMR' | int32 1 | int64 n | int32 nout | uint64 FNV-1a hash of the input file |
 //               double out[nout n]; qdim 3: potential (3) then curl field (3) per particle,
@@ -102,9 +103,11 @@ int main(int argc, char** argv)
 {
     std::vector<std::string> args;
     std::string in_path, out_path;
+    bool device_tree = false;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         if ((a == "--in" || a == "--out") && i + 1 < argc) (a == "--in" ? in_path : out_path) = argv[++i];
+        else if (a == "--device-tree") device_tree = true;
         else args.push_back(a);
     }
     size_t N = args.size() > 0 ? std::strtoull(args[0].c_str(), nullptr, 10) : 200000;
@@ -140,7 +143,7 @@ int main(int argc, char** argv)
     std::cout << box << std::endl;
 
     OctreePartitioner<double, Vector> partitioner(positions, box);
-    partitioner.refine(20, max_particles_per_node);
+    partitioner.refine(20, max_particles_per_node, device_tree);
     std::cout << partitioner << std::endl;
 
     int comm = 0;  // stands for the communicator argument; ignored

@@ -127,9 +127,11 @@ public:
         if (_tree) sfmm_tree_destroy(_tree);
     }
 
-    void refine(int max_level, int max_particles_per_node)
+    // partition.hpp:405-446; on_device builds the same tree on the GPU
+    void refine(int max_level, int max_particles_per_node, bool on_device = false)
     {
         sfmm_tree_desc d{};
+        d.on_device = on_device ? 1 : 0;
         d.n = static_cast<int64_t>(_positions.size());
         d.positions = reinterpret_cast<const double*>(_positions.data());
         d.max_level = max_level;

@@ -34,17 +34,17 @@ def _read_output(path):
     return h, out
 
 
-@pytest.mark.parametrize("qdim", [1, 3])
-def test_driver_files_match_binding(tmp_path, qdim):
+@pytest.mark.parametrize("qdim,device_tree", [(1, False), (3, False), (1, True)])
+def test_driver_files_match_binding(tmp_path, qdim, device_tree):
     assert os.path.exists(DRIVER), "build the driver: make -C simplefmm_amd/cpp"
     pos, q = W.clusters(6000, seed=71)
     if qdim == 3:
         q = np.random.default_rng(3).uniform(-1, 1, (6000, 3))
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     W.write_harness_input(str(fin), pos, q)
-    print(f"driver qdim={qdim}", flush=True)
-    r = subprocess.run([DRIVER, "0", "48", "200", "1", "--in", str(fin), "--out", str(fout)], capture_output=True,
-                       text=True, timeout=120)
+    print(f"driver qdim={qdim} device_tree={device_tree}", flush=True)
+    cmd = [DRIVER, "0", "48", "200", "1", "--in", str(fin), "--out", str(fout)] + (["--device-tree"] if device_tree else [])
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     h, out = _read_output(fout)
     assert h == _fnv1a(fin.read_bytes())
