@@ -91,7 +91,7 @@ __host__ __device__ constexpr int s1_koff(int q)
 {
     const auto S = schedule<P, SC>();
     int o = 0;
-    for (int i = 0; i < q; ++i) o += rot_nks(S.s1.item[W][i] >> 8, (S.s1.item[W][i] >> 4) & 15);
+    for (int i = 0; i < q; ++i) o += Packed<P>::s1.g[S.s1.item[W][i]].nks();
     return o;
 }
 template <int P, int SC, int W>
@@ -99,8 +99,44 @@ __host__ __device__ constexpr int s2_koff(int q)
 {
     const auto S = schedule<P, SC>();
     int o = 0;
-    for (int i = 0; i < q; ++i) o += coax_nks(P, S.s2.item[W][i] >> 8);
+    for (int i = 0; i < q; ++i) o += Packed<P>::s2.g[S.s2.item[W][i]].nks();
     return o;
+}
+
+// LDS rows of the packed groups (m2l_sched.hpp).  Stage 1 reads degree-major
+// X and writes order-major Y; stage 2 reads Y and writes degree-major X.
+// Columns past a group's pieces read the zero row K; rows past them write the
+// trash row.
+template <int P>
+__host__ __device__ constexpr int s1_in_row(int gi, int f)
+{
+    const Packing& pk = Packed<P>::s1;
+    const int q = pk.piece_of_col(gi, f);
+    if (q < 0) return (P + 1) * (P + 1);
+    const Piece& pc = pk.g[gi].pc[q];
+    const int c = f - pk.g[gi].col0[q];
+    return pc.part == 0 ? pc.n * pc.n + c : pc.n * pc.n + pc.n + 1 + c;
+}
+template <int P>
+__host__ __device__ constexpr int s1_out_row(int gi, int i);
+template <int P>
+__host__ __device__ constexpr int s2_in_row(int gi, int f)
+{
+    const Packing& pk = Packed<P>::s2;
+    const int q = pk.piece_of_col(gi, f);
+    if (q < 0) return (P + 1) * (P + 1);
+    const Piece& pc = pk.g[gi].pc[q];
+    return om_row(P, pc.n, pc.n + f - pk.g[gi].col0[q], pc.part);
+}
+template <int P>
+__host__ __device__ constexpr int s2_out_row(int gi, int i);
+
+// One of four compile-time rows by h = lane / 16 (a run of four when possible).
+template <int R0, int R1, int R2, int R3>
+__device__ __forceinline__ int sel_row(int h)
+{
+    if constexpr (R1 == R0 + 1 && R2 == R0 + 2 && R3 == R0 + 3) return R0 + h;
+    else return h < 2 ? (h == 0 ? R0 : R1) : (h == 2 ? R2 : R3);
 }
 template <int P, int SC, int W>
 __host__ __device__ constexpr int s3_koff(int q)
@@ -120,11 +156,10 @@ __device__ __forceinline__ void load_rot_fwd(T* a1, const T* A1, int lane)
     constexpr auto S = schedule<P, SC>();
     static_for<0, S.s1.count[W]>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int item = S.s1.item[W][q];
-        constexpr int n = item >> 8, part = (item >> 4) & 15, mt = item & 15;
-        constexpr int frag = rot_frag(n, part, mt), ko = s1_koff<P, SC, W>(q);
+        constexpr int g = S.s1.item[W][q];
+        constexpr int frag = Packed<P>::s1.frag(g), ko = s1_koff<P, SC, W>(q), nk = Packed<P>::s1.g[g].nks();
 #pragma unroll
-        for (int ks = 0; ks < rot_nks(n, part); ++ks) a1[ko + ks] = A1[(frag + ks) * 64 + lane];
+        for (int ks = 0; ks < nk; ++ks) a1[ko + ks] = A1[(frag + ks) * 64 + lane];
     });
 }
 
@@ -134,11 +169,10 @@ __device__ __forceinline__ void load_coax(T* a2, const T* A2, int lane)
     constexpr auto S = schedule<P, SC>();
     static_for<0, S.s2.count[W]>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int item = S.s2.item[W][q];
-        constexpr int k = item >> 8, mt = item & 15;
-        constexpr int frag = coax_frag(P, k, mt), ko = s2_koff<P, SC, W>(q);
+        constexpr int g = S.s2.item[W][q];
+        constexpr int frag = Packed<P>::s2.frag(g), ko = s2_koff<P, SC, W>(q), nk = Packed<P>::s2.g[g].nks();
 #pragma unroll
-        for (int ks = 0; ks < coax_nks(P, k); ++ks) a2[ko + ks] = A2[(frag + ks) * 64 + lane];
+        for (int ks = 0; ks < nk; ++ks) a2[ko + ks] = A2[(frag + ks) * 64 + lane];
     });
 }
 
@@ -180,6 +214,26 @@ __device__ __forceinline__ int ent_dist(int w) { return w >> 16; }
 // basic block the compiler can schedule freely.
 constexpr int kTrashRow = 0;  // relative to the end of the padded rows
 
+template <int P>
+__host__ __device__ constexpr int s1_out_row(int gi, int i)
+{
+    const Packing& pk = Packed<P>::s1;
+    const int q = pk.piece_of_row(gi, i);
+    if (q < 0) return (P + 1) * (P + 1) + kPadRows + kTrashRow;
+    const Piece& pc = pk.g[gi].pc[q];
+    const int a = pc.r0 + i - pk.g[gi].row0[q];  // sin rows start at order 1
+    return om_row(P, pc.part == 0 ? a : a + 1, pc.n, pc.part);
+}
+template <int P>
+__host__ __device__ constexpr int s2_out_row(int gi, int i)
+{
+    const Packing& pk = Packed<P>::s2;
+    const int q = pk.piece_of_row(gi, i);
+    if (q < 0) return (P + 1) * (P + 1) + kPadRows + kTrashRow;
+    const Piece& pc = pk.g[gi].pc[q];
+    return ps_row(pc.n + pc.r0 + i - pk.g[gi].row0[q], pc.n, pc.part);
+}
+
 // (cos, sin) pair of the azimuth table, read with one LDS access
 template <typename T>
 struct alignas(2 * sizeof(T)) CsPair {
@@ -212,15 +266,17 @@ template <typename T, int P, int SC, int W>
 __device__ __forceinline__ void read_rot_fwd(T* b, const T* X, int lane)
 {
     constexpr auto S = schedule<P, SC>();
-    const int col = lane & 15;
+    const int col = lane & 15, h = lane >> 4;
     static_for<0, S.s1.count[W]>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int item = S.s1.item[W][q];
-        constexpr int n = item >> 8, part = (item >> 4) & 15;
-        constexpr int brow = part == 0 ? n * n : n * n + n + 1;
+        constexpr int g = S.s1.item[W][q];
         constexpr int ko = s1_koff<P, SC, W>(q);
-#pragma unroll
-        for (int ks = 0; ks < rot_nks(n, part); ++ks) b[ko + ks] = X[(brow + 4 * ks + (lane >> 4)) * kLdsW + col];
+        static_for<0, Packed<P>::s1.g[g].nks()>([&](auto kc) {
+            constexpr int ks = decltype(kc)::value;
+            const int row = sel_row<s1_in_row<P>(g, 4 * ks), s1_in_row<P>(g, 4 * ks + 1), s1_in_row<P>(g, 4 * ks + 2),
+                                    s1_in_row<P>(g, 4 * ks + 3)>(h);
+            b[ko + ks] = X[row * kLdsW + col];
+        });
     });
 }
 
@@ -228,15 +284,17 @@ template <typename T, int P, int SC, int W>
 __device__ __forceinline__ void read_coax(T* b, const T* Y, int lane)
 {
     constexpr auto S = schedule<P, SC>();
-    const int col = lane & 15;
+    const int col = lane & 15, h = lane >> 4;
     static_for<0, S.s2.count[W]>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int item = S.s2.item[W][q];
-        constexpr int k = item >> 8, part = (item >> 4) & 15;
-        constexpr int brow = om_row(P, k, k, part);
+        constexpr int g = S.s2.item[W][q];
         constexpr int ko = s2_koff<P, SC, W>(q);
-#pragma unroll
-        for (int ks = 0; ks < coax_nks(P, k); ++ks) b[ko + ks] = Y[(brow + 4 * ks + (lane >> 4)) * kLdsW + col];
+        static_for<0, Packed<P>::s2.g[g].nks()>([&](auto kc) {
+            constexpr int ks = decltype(kc)::value;
+            const int row = sel_row<s2_in_row<P>(g, 4 * ks), s2_in_row<P>(g, 4 * ks + 1), s2_in_row<P>(g, 4 * ks + 2),
+                                    s2_in_row<P>(g, 4 * ks + 3)>(h);
+            b[ko + ks] = Y[row * kLdsW + col];
+        });
     });
 }
 
@@ -265,8 +323,8 @@ __host__ __device__ constexpr int s1_maxks()
     const auto S = schedule<P, SC>();
     int m = 0;
     for (int q = 0; q < S.s1.count[W]; ++q) {
-        const int it = S.s1.item[W][q];
-        m = rot_nks(it >> 8, (it >> 4) & 15) > m ? rot_nks(it >> 8, (it >> 4) & 15) : m;
+        const int nk = Packed<P>::s1.g[S.s1.item[W][q]].nks();
+        m = nk > m ? nk : m;
     }
     return m;
 }
@@ -275,7 +333,10 @@ __host__ __device__ constexpr int s2_maxks()
 {
     const auto S = schedule<P, SC>();
     int m = 0;
-    for (int q = 0; q < S.s2.count[W]; ++q) m = coax_nks(P, S.s2.item[W][q] >> 8) > m ? coax_nks(P, S.s2.item[W][q] >> 8) : m;
+    for (int q = 0; q < S.s2.count[W]; ++q) {
+        const int nk = Packed<P>::s2.g[S.s2.item[W][q]].nks();
+        m = nk > m ? nk : m;
+    }
     return m;
 }
 template <int P, int SC, int W>
@@ -307,21 +368,20 @@ __device__ __forceinline__ void stage_rot_fwd(const T* a1, const T* b, T* Y, int
         constexpr int ks = decltype(kc)::value;
         static_for<0, NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            constexpr int item = S.s1.item[W][q];
+            constexpr int g = S.s1.item[W][q];
             constexpr int ko = s1_koff<P, SC, W>(q);
-            if constexpr (ks < rot_nks(item >> 8, (item >> 4) & 15)) d[q] = F::run(a1[ko + ks], b[ko + ks], d[q]);
+            if constexpr (ks < Packed<P>::s1.g[g].nks()) d[q] = F::run(a1[ko + ks], b[ko + ks], d[q]);
         });
     });
+    const int h = lane >> 4;  // register r holds packed rows 4r + h
     static_for<0, NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int item = S.s1.item[W][q];
-        constexpr int n = item >> 8, part = (item >> 4) & 15, mt = item & 15;
+        constexpr int g = S.s1.item[W][q];
         static_for<0, 4>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
-            if constexpr (16 * mt + 4 * r <= n) {  // register r holds rows [4r, 4r + 4) of the tile
-                const int row = 16 * mt + F::drow(lane, r);
-                const bool ok = row <= n && (part == 0 || row > 0);
-                const int dst = ok ? om_row(P, row, n, part) : K + kPadRows + kTrashRow;
+            if constexpr (4 * r < Packed<P>::s1.g[g].rows) {
+                const int dst = sel_row<s1_out_row<P>(g, 4 * r), s1_out_row<P>(g, 4 * r + 1), s1_out_row<P>(g, 4 * r + 2),
+                                        s1_out_row<P>(g, 4 * r + 3)>(h);
                 Y[dst * kLdsW + col] = d[q][r];
             }
         });
@@ -345,19 +405,20 @@ __device__ __forceinline__ void stage_coax(const T* a2, const T* b, T* X, int la
         constexpr int ks = decltype(kc)::value;
         static_for<0, NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
+            constexpr int g = S.s2.item[W][q];
             constexpr int ko = s2_koff<P, SC, W>(q);
-            if constexpr (ks < coax_nks(P, S.s2.item[W][q] >> 8)) d[q] = F::run(a2[ko + ks], b[ko + ks], d[q]);
+            if constexpr (ks < Packed<P>::s2.g[g].nks()) d[q] = F::run(a2[ko + ks], b[ko + ks], d[q]);
         });
     });
+    const int h = lane >> 4;  // register r holds packed rows 4r + h
     static_for<0, NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int item = S.s2.item[W][q];
-        constexpr int k = item >> 8, part = (item >> 4) & 15, mt = item & 15;
+        constexpr int g = S.s2.item[W][q];
         static_for<0, 4>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
-            if constexpr (k + 16 * mt + 4 * r <= P) {
-                const int j = k + 16 * mt + F::drow(lane, r);
-                const int dst = j <= P ? ps_row(j, k, part) : K + kPadRows + kTrashRow;
+            if constexpr (4 * r < Packed<P>::s2.g[g].rows) {
+                const int dst = sel_row<s2_out_row<P>(g, 4 * r), s2_out_row<P>(g, 4 * r + 1), s2_out_row<P>(g, 4 * r + 2),
+                                        s2_out_row<P>(g, 4 * r + 3)>(h);
                 X[dst * kLdsW + col] = d[q][r];
             }
         });
@@ -431,7 +492,7 @@ struct WaveState {
     static constexpr int NT = (P + 1) * (P + 2) / 2;
     static constexpr int TB = 16 / NC;
     static constexpr int NRF = rot_nfrag(P);
-    static constexpr int NCF = coax_nfrag(P);
+    static constexpr int NCF = Packed<P>::s2.nfrag();  // packed stage-2 fragments per distance class
     static constexpr auto S = schedule<P, SC>();
     static constexpr int NJ = S.s3.count[W];
     static constexpr int NK1 = s1_koff<P, SC, W>(S.s1.count[W]);

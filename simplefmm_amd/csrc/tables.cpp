@@ -115,11 +115,34 @@ void build_m2l_fragments(const Tables& t, bool interleave_rows, std::vector<doub
 {
     auto mrow = [&](int l) { return interleave_rows ? 4 * (l % 4) + (l % 16) / 4 : l % 16; };
     const int P = t.p;
-    const int nrf = rot_nfrag(P), ncf = coax_nfrag(P);
+    const Packing p1 = pack_stage(P, 1), p2 = pack_stage(P, 2);
+    const int nrf = rot_nfrag(P), ncf = p2.nfrag();
     rotA.assign(static_cast<size_t>(70) * 2 * nrf * 64, 0.0);
     coaxA.assign(static_cast<size_t>(28) * ncf * 64, 0.0);
-    for (int cls = 0; cls < 70; ++cls)
-        for (int dir = 0; dir < 2; ++dir) {
+    for (int cls = 0; cls < 70; ++cls) {
+        // forward rotation (stage 1): packed groups; lane l holds A[l % 16][l / 16]
+        {
+            const double* blocks = t.rot.data() + static_cast<size_t>(cls) * 2 * t.rot_size;
+            double* out = rotA.data() + static_cast<size_t>(cls) * 2 * nrf * 64;
+            for (int g = 0; g < p1.ng; ++g)
+                for (int ks = 0; ks < p1.g[g].nks(); ++ks) {
+                    double* f = out + static_cast<size_t>(p1.frag(g) + ks) * 64;
+                    for (int l = 0; l < 64; ++l) {
+                        const int i = mrow(l), fc = 4 * ks + l / 16;
+                        const int qr = p1.piece_of_row(g, i), qc = p1.piece_of_col(g, fc);
+                        double v = 0.0;
+                        if (qr >= 0 && qr == qc) {
+                            const Piece& pc = p1.g[g].pc[qr];
+                            const int n = pc.n;
+                            const int a = pc.r0 + i - p1.g[g].row0[qr], b = fc - p1.g[g].col0[qr];
+                            v = pc.part == 0 ? blocks[t.rot_off[n] + a * (n + 1) + b]
+                                             : blocks[t.rot_off[n] + (n + 1) * (n + 1) + a * n + b];
+                        }
+                        f[l] = v;
+                    }
+                }
+        }
+        for (int dir = 1; dir < 2; ++dir) {
             const double* blocks = t.rot.data() + (static_cast<size_t>(cls) * 2 + dir) * t.rot_size;
             double* out = rotA.data() + (static_cast<size_t>(cls) * 2 + dir) * nrf * 64;
             for (int n = 0; n <= P; ++n)
@@ -140,20 +163,27 @@ void build_m2l_fragments(const Tables& t, bool interleave_rows, std::vector<doub
                         }
                 }
         }
+    }
+    // coaxial translation (stage 2): packed groups, the block of order k once per part
     for (int d2 = 0; d2 < 28; ++d2) {
         const double* C = t.coax.data() + static_cast<size_t>(d2) * t.coax_size;
         double* out = coaxA.data() + static_cast<size_t>(d2) * ncf * 64;
-        for (int k = 0; k <= P; ++k) {
-            const int w = P + 1 - k;
-            for (int mt = 0; mt < coax_mts(P, k); ++mt)
-                for (int ks = 0; ks < coax_nks(P, k); ++ks) {
-                    double* f = out + static_cast<size_t>(coax_frag(P, k, mt) + ks) * 64;
-                    for (int l = 0; l < 64; ++l) {
-                        const int row = 16 * mt + mrow(l), kk = 4 * ks + l / 16;
-                        f[l] = (row < w && kk < w) ? C[t.coax_off[k] + row * w + kk] : 0.0;
+        for (int g = 0; g < p2.ng; ++g)
+            for (int ks = 0; ks < p2.g[g].nks(); ++ks) {
+                double* f = out + static_cast<size_t>(p2.frag(g) + ks) * 64;
+                for (int l = 0; l < 64; ++l) {
+                    const int i = mrow(l), fc = 4 * ks + l / 16;
+                    const int qr = p2.piece_of_row(g, i), qc = p2.piece_of_col(g, fc);
+                    double v = 0.0;
+                    if (qr >= 0 && qr == qc) {
+                        const Piece& pc = p2.g[g].pc[qr];
+                        const int k = pc.n, w = P + 1 - k;
+                        const int row = pc.r0 + i - p2.g[g].row0[qr], kk = fc - p2.g[g].col0[qr];
+                        v = C[t.coax_off[k] + row * w + kk];
                     }
+                    f[l] = v;
                 }
-        }
+            }
     }
 }
 
