@@ -131,6 +131,31 @@ __host__ __device__ constexpr int s2_in_row(int gi, int f)
 template <int P>
 __host__ __device__ constexpr int s2_out_row(int gi, int i);
 
+// Stage 3 input row of packed column f of group gi's cos (part 0) or sin (part 1)
+// tile; degree j reads X rows j^2 + b (cos) and j^2 + j + b (sin, b >= 1).
+template <int P>
+__host__ __device__ constexpr int s3_in_row(int gi, int part, int f)
+{
+    const Packing& pk = Packed<P>::s3;
+    const int q = part == 0 ? pk.piece_of_col(gi, f) : pk.piece_of_col_s(gi, f);
+    if (q < 0) return (P + 1) * (P + 1);
+    const int j = pk.g[gi].pc[q].n;
+    return part == 0 ? j * j + f - pk.g[gi].col0[q] : j * j + j + 1 + f - pk.g[gi].col0s[q];
+}
+// Degree (or -1) and order of packed row i of stage-3 group gi.
+template <int P>
+__host__ __device__ constexpr int s3_row_j(int gi, int i)
+{
+    const int q = Packed<P>::s3.piece_of_row(gi, i);
+    return q < 0 ? -1 : Packed<P>::s3.g[gi].pc[q].n;
+}
+template <int P>
+__host__ __device__ constexpr int s3_row_a(int gi, int i)
+{
+    const int q = Packed<P>::s3.piece_of_row(gi, i);
+    return q < 0 ? 0 : Packed<P>::s3.g[gi].pc[q].r0 + i - Packed<P>::s3.g[gi].row0[q];
+}
+
 // One of four compile-time rows by h = lane / 16 (a run of four when possible).
 template <int R0, int R1, int R2, int R3>
 __device__ __forceinline__ int sel_row(int h)
@@ -144,8 +169,8 @@ __host__ __device__ constexpr int s3_koff(int q)
     const auto S = schedule<P, SC>();
     int o = 0;
     for (int i = 0; i < q; ++i) {
-        const int j = S.s3.item[W][i] >> 8;
-        o += rot_mts(j) * (rot_nks(j, 0) + rot_nks(j, 1));
+        const Group& G = Packed<P>::s3.g[S.s3.item[W][i]];
+        o += G.nks() + G.nks_s();
     }
     return o;
 }
@@ -182,17 +207,11 @@ __device__ __forceinline__ void load_rot_bwd(T* a3, const T* A3, int lane)
     constexpr auto S = schedule<P, SC>();
     static_for<0, S.s3.count[W]>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int j = S.s3.item[W][q] >> 8;
-        constexpr int ko = s3_koff<P, SC, W>(q);
-        constexpr int nkc = rot_nks(j, 0), nks = rot_nks(j, 1);
+        constexpr int g = S.s3.item[W][q];
+        constexpr int ko = s3_koff<P, SC, W>(q), frag = Packed<P>::s3.frag3(g);
+        constexpr int nk = Packed<P>::s3.g[g].nks() + Packed<P>::s3.g[g].nks_s();
 #pragma unroll
-        for (int mt = 0; mt < rot_mts(j); ++mt) {
-#pragma unroll
-            for (int ks = 0; ks < nkc; ++ks) a3[ko + mt * (nkc + nks) + ks] = A3[(rot_frag(j, 0, mt) + ks) * 64 + lane];
-#pragma unroll
-            for (int ks = 0; ks < nks; ++ks)
-                a3[ko + mt * (nkc + nks) + nkc + ks] = A3[(rot_frag(j, 1, mt) + ks) * 64 + lane];
-        }
+        for (int ks = 0; ks < nk; ++ks) a3[ko + ks] = A3[(frag + ks) * 64 + lane];
     });
 }
 
@@ -244,13 +263,7 @@ struct alignas(2 * sizeof(T)) CsPair {
 template <int P, int SC, int W>
 __host__ __device__ constexpr int s3_boff(int q)
 {
-    const auto S = schedule<P, SC>();
-    int o = 0;
-    for (int i = 0; i < q; ++i) {
-        const int j = S.s3.item[W][i] >> 8;
-        o += rot_nks(j, 0) + rot_nks(j, 1);
-    }
-    return o;
+    return s3_koff<P, SC, W>(q);
 }
 
 template <typename T, int P, int SC, int W>
@@ -302,17 +315,24 @@ template <typename T, int P, int SC, int W>
 __device__ __forceinline__ void read_rot_back(T* b, const T* X, int lane)
 {
     constexpr auto S = schedule<P, SC>();
-    const int col = lane & 15;
+    const int col = lane & 15, h = lane >> 4;
     static_for<0, S.s3.count[W]>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int j = S.s3.item[W][q] >> 8;
+        constexpr int g = S.s3.item[W][q];
         constexpr int bo = s3_boff<P, SC, W>(q);
-        constexpr int nkc = rot_nks(j, 0);
-#pragma unroll
-        for (int ks = 0; ks < nkc; ++ks) b[bo + ks] = X[(j * j + 4 * ks + (lane >> 4)) * kLdsW + col];
-#pragma unroll
-        for (int ks = 0; ks < rot_nks(j, 1); ++ks)
-            b[bo + nkc + ks] = X[(j * j + j + 1 + 4 * ks + (lane >> 4)) * kLdsW + col];
+        constexpr int nkc = Packed<P>::s3.g[g].nks();
+        static_for<0, nkc>([&](auto kc) {
+            constexpr int ks = decltype(kc)::value;
+            const int row = sel_row<s3_in_row<P>(g, 0, 4 * ks), s3_in_row<P>(g, 0, 4 * ks + 1),
+                                    s3_in_row<P>(g, 0, 4 * ks + 2), s3_in_row<P>(g, 0, 4 * ks + 3)>(h);
+            b[bo + ks] = X[row * kLdsW + col];
+        });
+        static_for<0, Packed<P>::s3.g[g].nks_s()>([&](auto kc) {
+            constexpr int ks = decltype(kc)::value;
+            const int row = sel_row<s3_in_row<P>(g, 1, 4 * ks), s3_in_row<P>(g, 1, 4 * ks + 1),
+                                    s3_in_row<P>(g, 1, 4 * ks + 2), s3_in_row<P>(g, 1, 4 * ks + 3)>(h);
+            b[bo + nkc + ks] = X[row * kLdsW + col];
+        });
     });
 }
 
@@ -344,7 +364,10 @@ __host__ __device__ constexpr int s3_maxks()
 {
     const auto S = schedule<P, SC>();
     int m = 0;
-    for (int q = 0; q < S.s3.count[W]; ++q) m = rot_nks(S.s3.item[W][q] >> 8, 0) > m ? rot_nks(S.s3.item[W][q] >> 8, 0) : m;
+    for (int q = 0; q < S.s3.count[W]; ++q) {
+        const int nk = Packed<P>::s3.g[S.s3.item[W][q]].nks();
+        m = nk > m ? nk : m;
+    }
     return m;
 }
 
@@ -435,44 +458,34 @@ __device__ __forceinline__ void stage_rot_back(const T* a3, const T* b, const T*
     using V4 = typename F::V4;
     constexpr auto S = schedule<P, SC>();
     constexpr int NQ = S.s3.count[W];
-    constexpr int MT = rot_mts(P);
-    V4 dc[NQ > 0 ? NQ : 1][MT], ds[NQ > 0 ? NQ : 1][MT];
+    V4 dc[NQ > 0 ? NQ : 1], ds[NQ > 0 ? NQ : 1];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q)
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) dc[q][mt] = ds[q][mt] = V4{0, 0, 0, 0};
+    for (int q = 0; q < NQ; ++q) dc[q] = ds[q] = V4{0, 0, 0, 0};
     static_for<0, s3_maxks<P, SC, W>()>([&](auto kc) {
         constexpr int ks = decltype(kc)::value;
         static_for<0, NQ>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            constexpr int j = S.s3.item[W][q] >> 8;
+            constexpr int g = S.s3.item[W][q];
             constexpr int ko = s3_koff<P, SC, W>(q);
-            constexpr int bo = s3_boff<P, SC, W>(q);
-            constexpr int nkc = rot_nks(j, 0), nks = rot_nks(j, 1);
-            static_for<0, rot_mts(j)>([&](auto mc) {
-                constexpr int mt = decltype(mc)::value;
-                constexpr int kb = ko + mt * (nkc + nks);
-                if constexpr (ks < nkc) dc[q][mt] = F::run(a3[kb + ks], b[bo + ks], dc[q][mt]);
-                if constexpr (ks < nks) ds[q][mt] = F::run(a3[kb + nkc + ks], b[bo + nkc + ks], ds[q][mt]);
-            });
+            constexpr int nkc = Packed<P>::s3.g[g].nks(), nks = Packed<P>::s3.g[g].nks_s();
+            if constexpr (ks < nkc) dc[q] = F::run(a3[ko + ks], b[ko + ks], dc[q]);
+            if constexpr (ks < nks) ds[q] = F::run(a3[ko + nkc + ks], b[ko + nkc + ks], ds[q]);
         });
     });
+    const int h = lane >> 4;  // register r holds packed rows 4r + h
     static_for<0, NQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int j = S.s3.item[W][q] >> 8;
-        static_for<0, rot_mts(j)>([&](auto mc) {
-            constexpr int mt = decltype(mc)::value;
-            static_for<0, 4>([&](auto rc_) {
-                constexpr int r = decltype(rc_)::value;
-                if constexpr (16 * mt + 4 * r <= j) {  // registers past the degree hold zeros
-                    const int row = 16 * mt + F::drow(lane, r);
-                    const int rc = row <= j ? row : j;
-                    const CsPair<T> e = reinterpret_cast<const CsPair<T>*>(cs)[rc];
-                    const T lc = dc[q][mt][r], ls = ds[q][mt][r];
-                    acc[q][mt][0][r] += static_cast<double>(lc * e.c + ls * e.s);
-                    acc[q][mt][1][r] += static_cast<double>(ls * e.c - lc * e.s);
-                }
-            });
+        constexpr int g = S.s3.item[W][q];
+        static_for<0, 4>([&](auto rc_) {
+            constexpr int r = decltype(rc_)::value;
+            if constexpr (4 * r < Packed<P>::s3.g[g].rows) {  // rows past the pieces hold zeros
+                const int m = sel_row<s3_row_a<P>(g, 4 * r), s3_row_a<P>(g, 4 * r + 1), s3_row_a<P>(g, 4 * r + 2),
+                                      s3_row_a<P>(g, 4 * r + 3)>(h);
+                const CsPair<T> e = reinterpret_cast<const CsPair<T>*>(cs)[m];
+                const T lc = dc[q][r], ls = ds[q][r];
+                acc[q][0][0][r] += static_cast<double>(lc * e.c + ls * e.s);
+                acc[q][0][1][r] += static_cast<double>(ls * e.c - lc * e.s);
+            }
         });
     });
 }
@@ -619,11 +632,10 @@ struct Acc {
 template <typename T, int P, int NC, int SC, int W, class A>
 __device__ __forceinline__ void write_locals(const M2LArgs& a, int64_t blk, double* __restrict__ L, A& acc, int lane)
 {
-    using F = Mfma<T>;
     constexpr int K = (P + 1) * (P + 1);
     constexpr int TB = 16 / NC;
     constexpr auto S = schedule<P, SC>();
-    const int col = lane & 15;
+    const int col = lane & 15, h = lane >> 4;
     const int t = col / NC, c = col - (col / NC) * NC;
     if (t >= TB) return;
     const int32_t tg = a.blk_tgt[blk * TB + t];
@@ -631,17 +643,20 @@ __device__ __forceinline__ void write_locals(const M2LArgs& a, int64_t blk, doub
     double* Lt = L + (static_cast<int64_t>(tg) * NC + c) * K;
     static_for<0, S.s3.count[W]>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int j = S.s3.item[W][q] >> 8;
-#pragma unroll
-        for (int mt = 0; mt < rot_mts(j); ++mt)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = 16 * mt + F::drow(lane, r);
-                if (row <= j) {
-                    Lt[rfc2(j, row)] = acc[q][mt][0][r];
-                    if (row > 0) Lt[rfs2(j, row)] = acc[q][mt][1][r];
+        constexpr int g = S.s3.item[W][q];
+        static_for<0, 4>([&](auto rc_) {
+            constexpr int r = decltype(rc_)::value;
+            if constexpr (4 * r < Packed<P>::s3.g[g].rows) {
+                const int j = sel_row<s3_row_j<P>(g, 4 * r), s3_row_j<P>(g, 4 * r + 1), s3_row_j<P>(g, 4 * r + 2),
+                                      s3_row_j<P>(g, 4 * r + 3)>(h);
+                const int m = sel_row<s3_row_a<P>(g, 4 * r), s3_row_a<P>(g, 4 * r + 1), s3_row_a<P>(g, 4 * r + 2),
+                                      s3_row_a<P>(g, 4 * r + 3)>(h);
+                if (j >= 0) {
+                    Lt[rfc2(j, m)] = acc[q][0][0][r];
+                    if (m > 0) Lt[rfs2(j, m)] = acc[q][0][1][r];
                 }
             }
+        });
     });
 }
 
@@ -707,7 +722,7 @@ __device__ __forceinline__ void m2l_pipe_wave(const M2LArgs& a, int64_t blk, int
     auto ce = [&](int k) { return k < 0 ? 0 : (k >= ne ? ne - 1 : k); };
     auto cs_slot = [&](int k) { return cst + (k & (kCsSlots - 1)) * NCS; };
 
-    Acc<WS::NJ, rot_mts(P)> acc;
+    Acc<WS::NJ, 1> acc;
     acc.zero();
     WS ws;
     ws.init(tid);
@@ -862,7 +877,7 @@ __device__ __forceinline__ void m2l_seq_wave(const M2LArgs& a, int64_t blk, int 
     constexpr int NCS = 2 * (P + 1);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    Acc<WS::NJ, rot_mts(P)> acc;
+    Acc<WS::NJ, 1> acc;
     acc.zero();
     WS ws;
     ws.init(tid);

@@ -75,7 +75,11 @@ struct Group {
     int np = 0, rows = 0, cols = 0;
     Piece pc[kMaxPieces] = {};
     int row0[kMaxPieces] = {}, col0[kMaxPieces] = {};
+    // stage 3 only: the sin tile of the group (same rows, columns b = 1..j per piece)
+    int cols_s = 0;
+    int col0s[kMaxPieces] = {};
     __host__ __device__ constexpr int nks() const { return ceil_div(cols, 4); }
+    __host__ __device__ constexpr int nks_s() const { return ceil_div(cols_s, 4); }
 };
 struct Packing {
     int ng = 0;
@@ -99,6 +103,19 @@ struct Packing {
         for (int q = 0; q < g[gi].np; ++q)
             if (f >= g[gi].col0[q] && f < g[gi].col0[q] + g[gi].pc[q].cols) return q;
         return -1;
+    }
+    __host__ __device__ constexpr int piece_of_col_s(int gi, int f) const
+    {
+        for (int q = 0; q < g[gi].np; ++q)
+            if (f >= g[gi].col0s[q] && f < g[gi].col0s[q] + g[gi].pc[q].n) return q;
+        return -1;
+    }
+    // stage 3: fragments of group gi (cos k-steps, then sin k-steps)
+    __host__ __device__ constexpr int frag3(int gi) const
+    {
+        int f = 0;
+        for (int i = 0; i < gi; ++i) f += g[i].nks() + g[i].nks_s();
+        return f;
     }
 };
 
@@ -147,10 +164,55 @@ __host__ __device__ constexpr Packing pack_stage(int P, int stage)
     return pk;
 }
 
+// Stage 3 (back rotation of degree j, then the azimuth rotation that pairs the
+// cos and sin rows of each order a): a piece is rows a = r0.. of degree j; the
+// group's cos tile and sin tile share its row layout (the sin row a = 0 is
+// zero) and have their own column layouts (cos b = 0..j, sin b = 1..j).
+__host__ __device__ constexpr Packing pack_stage3(int P)
+{
+    Piece pcs[2 * 18] = {};
+    int np = 0;
+    for (int j = 0; j <= P; ++j)
+        for (int r0 = 0; r0 <= j; r0 += 16) {
+            Piece q;
+            q.n = j;
+            q.r0 = r0;
+            q.rows = j + 1 - r0 < 16 ? j + 1 - r0 : 16;
+            q.cols = j + 1;
+            pcs[np++] = q;
+        }
+    for (int i = 1; i < np; ++i) {
+        const Piece x = pcs[i];
+        int k = i - 1;
+        while (k >= 0 && pcs[k].rows < x.rows) {
+            pcs[k + 1] = pcs[k];
+            --k;
+        }
+        pcs[k + 1] = x;
+    }
+    Packing pk;
+    for (int i = 0; i < np; ++i) {
+        int gi = 0;
+        while (gi < pk.ng && (pk.g[gi].rows + pcs[i].rows > 16 || pk.g[gi].np == kMaxPieces)) ++gi;
+        if (gi == pk.ng) ++pk.ng;
+        Group& G = pk.g[gi];
+        G.pc[G.np] = pcs[i];
+        G.row0[G.np] = G.rows;
+        G.col0[G.np] = G.cols;
+        G.col0s[G.np] = G.cols_s;
+        G.rows += pcs[i].rows;
+        G.cols += pcs[i].cols;
+        G.cols_s += pcs[i].n;
+        ++G.np;
+    }
+    return pk;
+}
+
 template <int P>
 struct Packed {
     static constexpr Packing s1 = pack_stage(P, 1);
     static constexpr Packing s2 = pack_stage(P, 2);
+    static constexpr Packing s3 = pack_stage3(P);
 };
 
 // Static work split over the NW wavefronts of a workgroup: longest tiles
@@ -165,7 +227,7 @@ struct WaveTiles {
 
 template <int P, int NW, bool PIPE>
 struct M2LSchedule {
-    WaveTiles s1, s2, s3;  // s1, s2 items: packed group index; s3 items: degree j << 8 (both parts, all m-tiles)
+    WaveTiles s1, s2, s3;  // items: packed group index (stage 3: cos and sin tiles of the group)
 
     constexpr M2LSchedule()
     {
@@ -183,10 +245,11 @@ struct M2LSchedule {
             it[nt] = g;
             cost[nt++] = p2.g[g].nks();
         }
-        for (int j = P; j >= 0; --j) {
+        const Packing p3 = pack_stage3(P);
+        for (int g = 0; g < p3.ng; ++g) {
             st[nt] = 3;
-            it[nt] = j << 8;
-            cost[nt++] = rot_mts(j) * (rot_nks(j, 0) + rot_nks(j, 1));
+            it[nt] = g;
+            cost[nt++] = p3.g[g].nks() + p3.g[g].nks_s();
         }
         int load[3][kMaxWaves] = {};
         bool done[3 * 64] = {};

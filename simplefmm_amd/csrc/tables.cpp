@@ -115,7 +115,7 @@ void build_m2l_fragments(const Tables& t, bool interleave_rows, std::vector<doub
 {
     auto mrow = [&](int l) { return interleave_rows ? 4 * (l % 4) + (l % 16) / 4 : l % 16; };
     const int P = t.p;
-    const Packing p1 = pack_stage(P, 1), p2 = pack_stage(P, 2);
+    const Packing p1 = pack_stage(P, 1), p2 = pack_stage(P, 2), p3 = pack_stage3(P);
     const int nrf = rot_nfrag(P), ncf = p2.nfrag();
     rotA.assign(static_cast<size_t>(70) * 2 * nrf * 64, 0.0);
     coaxA.assign(static_cast<size_t>(28) * ncf * 64, 0.0);
@@ -142,26 +142,32 @@ void build_m2l_fragments(const Tables& t, bool interleave_rows, std::vector<doub
                     }
                 }
         }
-        for (int dir = 1; dir < 2; ++dir) {
-            const double* blocks = t.rot.data() + (static_cast<size_t>(cls) * 2 + dir) * t.rot_size;
-            double* out = rotA.data() + (static_cast<size_t>(cls) * 2 + dir) * nrf * 64;
-            for (int n = 0; n <= P; ++n)
-                for (int part = 0; part < 2; ++part) {
-                    if (part == 1 && n == 0) continue;
-                    for (int mt = 0; mt < rot_mts(n); ++mt)
-                        for (int ks = 0; ks < rot_nks(n, part); ++ks) {
-                            double* f = out + static_cast<size_t>(rot_frag(n, part, mt) + ks) * 64;
-                            for (int l = 0; l < 64; ++l) {
-                                const int row = 16 * mt + mrow(l), kk = 4 * ks + l / 16;
-                                double v = 0.0;
-                                if (part == 0 && row <= n && kk <= n)
-                                    v = blocks[t.rot_off[n] + row * (n + 1) + kk];
-                                else if (part == 1 && row >= 1 && row <= n && kk < n)
-                                    v = blocks[t.rot_off[n] + (n + 1) * (n + 1) + (row - 1) * n + kk];
-                                f[l] = v;
+        // back rotation (stage 3): packed groups, the cos tile's k-steps then the sin tile's
+        {
+            const double* blocks = t.rot.data() + (static_cast<size_t>(cls) * 2 + 1) * t.rot_size;
+            double* out = rotA.data() + (static_cast<size_t>(cls) * 2 + 1) * nrf * 64;
+            for (int g = 0; g < p3.ng; ++g) {
+                const int nkc = p3.g[g].nks();
+                for (int part = 0; part < 2; ++part)
+                    for (int ks = 0; ks < (part == 0 ? nkc : p3.g[g].nks_s()); ++ks) {
+                        double* f = out + static_cast<size_t>(p3.frag3(g) + part * nkc + ks) * 64;
+                        for (int l = 0; l < 64; ++l) {
+                            const int i = mrow(l), fc = 4 * ks + l / 16;
+                            const int qr = p3.piece_of_row(g, i);
+                            const int qc = part == 0 ? p3.piece_of_col(g, fc) : p3.piece_of_col_s(g, fc);
+                            double v = 0.0;
+                            if (qr >= 0 && qr == qc) {
+                                const Piece& pc = p3.g[g].pc[qr];
+                                const int j = pc.n, a = pc.r0 + i - p3.g[g].row0[qr];
+                                if (part == 0)
+                                    v = blocks[t.rot_off[j] + a * (j + 1) + fc - p3.g[g].col0[qr]];
+                                else if (a >= 1)
+                                    v = blocks[t.rot_off[j] + (j + 1) * (j + 1) + (a - 1) * j + fc - p3.g[g].col0s[qr]];
                             }
+                            f[l] = v;
                         }
-                }
+                    }
+            }
         }
     }
     // coaxial translation (stage 2): packed groups, the block of order k once per part
