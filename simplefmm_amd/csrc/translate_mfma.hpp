@@ -86,6 +86,19 @@ __device__ __forceinline__ void tr_static_for(F&& f)
     }
 }
 
+// longest k-range among wave W's m-tiles
+template <int P, bool UPPER, int W>
+__host__ __device__ constexpr int tr_maxk()
+{
+    constexpr TrTiles S = tr_schedule<P, UPPER>();
+    int m = 0;
+    for (int q = 0; q < S.count[W]; ++q) {
+        const int k = tr_ke(P, UPPER, S.mt[W][q]) - tr_kb(P, UPPER, S.mt[W][q]);
+        m = k > m ? k : m;
+    }
+    return m;
+}
+
 template <int P, int NC, bool UPPER, int W>
 __device__ __forceinline__ void translate_wave(const TransArgs& a, int64_t blk, double* __restrict__ io, double* Xs)
 {
@@ -118,16 +131,22 @@ __device__ __forceinline__ void translate_wave(const TransArgs& a, int64_t blk, 
         }
         __syncthreads();
         const double* A = a.frag + static_cast<int64_t>(oct) * NF * 64;
-        tr_static_for<0, NT>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            constexpr int mt = S.mt[W][q];
-            constexpr int kb = tr_kb(P, UPPER, mt), ke = tr_ke(P, UPPER, mt), f0 = tr_frag(P, UPPER, mt);
-#pragma unroll 4
-            for (int ks = kb; ks < ke; ++ks)
-                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(f0 + ks - kb) * 64 + lane],
-                                                             Xs[(4 * ks + (lane >> 4)) * kTrLdsW + col], acc[q], 0, 0,
-                                                             0);
-        });
+        // k-step by k-step across the wave's tiles, two steps per iteration, so
+        // that consecutive MFMAs are independent (each tile's chain keeps its order)
+        constexpr int KM = tr_maxk<P, UPPER, W>();
+        for (int kk0 = 0; kk0 < KM; kk0 += 2) {
+            tr_static_for<0, 2>([&](auto uc) {
+                const int kk = kk0 + decltype(uc)::value;
+                tr_static_for<0, NT>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    constexpr int mt = S.mt[W][q];
+                    constexpr int kb = tr_kb(P, UPPER, mt), ke = tr_ke(P, UPPER, mt), f0 = tr_frag(P, UPPER, mt);
+                    if (kk < ke - kb)
+                        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                            A[(f0 + kk) * 64 + lane], Xs[(4 * (kb + kk) + (lane >> 4)) * kTrLdsW + col], acc[q], 0, 0, 0);
+                });
+            });
+        }
     }
     const int t = col / NC, c = col - (col / NC) * NC;
     if (t >= TB) return;
