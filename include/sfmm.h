@@ -156,15 +156,16 @@ typedef struct {
 
 /* Prepares every plan for its item and assigns the items to concurrent streams
  * (largest first).  streams: 1..8, or 0 to time evaluations with 1, 2, 4, 8
- * streams (into scratch outputs) and keep the fastest.  The items' arrays must
- * stay valid while the batch is used. */
+ * streams and with every item on its plan's own stream (into scratch outputs)
+ * and keep the fastest.  The items' arrays must stay valid while the batch is used. */
 int sfmm_batch_create(const sfmm_batch_item* items, int count, int streams, sfmm_batch* out);
 /* Evaluates every item after the work queued on `stream` (hipStream_t; NULL: the
  * batch's own stream), which then waits for all of them; asynchronous. */
 int sfmm_batch_launch(sfmm_batch batch, void* stream);
 /* Waits for launches on the batch's own stream. */
 int sfmm_batch_synchronize(sfmm_batch batch);
-/* The number of concurrent streams of the batch. */
+/* The number of concurrent streams of the batch (the item count when every item
+ * runs on its plan's own stream). */
 int sfmm_batch_streams(sfmm_batch batch, int* streams);
 int sfmm_batch_destroy(sfmm_batch batch);
 

@@ -148,11 +148,14 @@ struct sfmm_batch_s {
     std::vector<hipStream_t> streams;  // [0] is the batch's own stream
     std::vector<hipEvent_t> fork, join;
     int nstream = 1;
+    bool own = false;               // every item on its plan's own stream
+    std::vector<hipEvent_t> pjoin;  // per item, for `own`
 
     ~sfmm_batch_s()
     {
         for (auto e : fork) (void)hipEventDestroy(e);
         for (auto e : join) (void)hipEventDestroy(e);
+        for (auto e : pjoin) (void)hipEventDestroy(e);
         for (auto st : streams) (void)hipStreamDestroy(st);
     }
 };
@@ -521,12 +524,25 @@ std::vector<int> batch_assign(const std::vector<sfmm_batch_item>& items, int ns)
     return out;
 }
 
-// Issues the items' evaluations on ns streams after the work already queued on
-// `origin`, and makes `origin` wait for all of them.
+// Issues the items' evaluations on ns streams (own: each on its plan's stream)
+// after the work already queued on `origin`, and makes `origin` wait for all of them.
 int batch_issue(sfmm_batch_s* B, const std::vector<sfmm_batch_item>& items, const std::vector<int>& where, int ns,
-                hipStream_t origin)
+                hipStream_t origin, bool own = false)
 {
     HIP_TRY(hipEventRecord(B->fork[0], origin));
+    if (own) {
+        for (size_t i = 0; i < items.size(); ++i) {
+            const sfmm_batch_item& it = items[i];
+            HIP_TRY(hipStreamWaitEvent(it.plan->stream, B->fork[0], 0));
+            sfmm::Mode mode;
+            int qdim, nc;
+            if (int r = mode_of(it.kind, it.ncharge, mode, qdim, nc)) return r;
+            if (int r = run_on(it.plan, mode, nc, qdim, it.dQ, it.dU, it.which, it.plan->stream, false)) return r;
+            HIP_TRY(hipEventRecord(B->pjoin[i], it.plan->stream));
+            HIP_TRY(hipStreamWaitEvent(origin, B->pjoin[i], 0));
+        }
+        return SFMM_OK;
+    }
     for (int k = 0; k < ns; ++k) HIP_TRY(hipStreamWaitEvent(B->streams[k], B->fork[0], 0));
     for (size_t i = 0; i < items.size(); ++i) {
         const sfmm_batch_item& it = items[i];
@@ -1138,6 +1154,8 @@ int sfmm_batch_create(const sfmm_batch_item* items, int count, int streams, sfmm
     B->join.resize(kBatchMaxStreams);
     for (auto& e : B->fork) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& e : B->join) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    B->pjoin.resize(count);
+    for (auto& e : B->pjoin) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
     int ns = std::min(streams, count);
     if (streams == 0) {
@@ -1160,20 +1178,31 @@ int sfmm_batch_create(const sfmm_batch_item* items, int count, int streams, sfmm
         hipStream_t s0 = B->streams[0];
         float best = -1.0f;
         int rc = SFMM_OK;
-        for (int cand = 1; cand <= std::min(count, kBatchMaxStreams) && rc == SFMM_OK; cand *= 2) {
-            const std::vector<int> where = batch_assign(trial, cand);
-            if ((rc = batch_issue(B.get(), trial, where, cand, s0)) != SFMM_OK) break;  // warm-up
-            float ms = 0.0f;
-            hipError_t e = hipEventRecord(t0, s0);
-            for (int r = 0; r < 2 && e == hipSuccess && rc == SFMM_OK; ++r) rc = batch_issue(B.get(), trial, where, cand, s0);
-            if (e == hipSuccess) e = hipEventRecord(t1, s0);
-            if (e == hipSuccess) e = hipEventSynchronize(t1);
-            if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        // each candidate: one warm-up launch, then the best of three timed launches;
+        // more streams are taken only when at least 2% faster (streams beyond the
+        // hardware queues share them)
+        // the last candidate issues every item on its plan's own stream
+        for (int cand = 1; cand <= 2 * std::min(count, kBatchMaxStreams) && rc == SFMM_OK; cand *= 2) {
+            const bool own = cand > std::min(count, kBatchMaxStreams);
+            const std::vector<int> where = batch_assign(trial, own ? 1 : cand);
+            if ((rc = batch_issue(B.get(), trial, where, cand, s0, own)) != SFMM_OK) break;  // warm-up
+            float cbest = -1.0f;
+            hipError_t e = hipSuccess;
+            for (int r = 0; r < 3 && e == hipSuccess && rc == SFMM_OK; ++r) {
+                float ms = 0.0f;
+                e = hipEventRecord(t0, s0);
+                if (e == hipSuccess) rc = batch_issue(B.get(), trial, where, cand, s0, own);
+                if (e == hipSuccess) e = hipEventRecord(t1, s0);
+                if (e == hipSuccess) e = hipEventSynchronize(t1);
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+                if (e == hipSuccess && (cbest < 0.0f || ms < cbest)) cbest = ms;
+            }
             if (rc != SFMM_OK) break;
             if (e != hipSuccess) rc = fail(SFMM_ERR_HIP, std::string("batch tuning: ") + hipGetErrorString(e));
-            else if (best < 0.0f || ms < best) {
-                best = ms;
-                ns = cand;
+            else if (best < 0.0f || cbest < 0.98f * best) {
+                best = cbest;
+                ns = own ? 1 : cand;
+                B->own = own;
             }
         }
         (void)hipStreamSynchronize(s0);
@@ -1193,7 +1222,7 @@ int sfmm_batch_launch(sfmm_batch batch, void* stream)
     if (!batch) return fail(SFMM_ERR_ARG, "null argument");
     HIP_TRY(hipSetDevice(batch->device));
     return batch_issue(batch, batch->items, batch->where, batch->nstream,
-                       stream ? static_cast<hipStream_t>(stream) : batch->streams[0]);
+                       stream ? static_cast<hipStream_t>(stream) : batch->streams[0], batch->own);
 }
 
 int sfmm_batch_synchronize(sfmm_batch batch)
@@ -1207,7 +1236,7 @@ int sfmm_batch_synchronize(sfmm_batch batch)
 int sfmm_batch_streams(sfmm_batch batch, int* streams)
 {
     if (!batch || !streams) return fail(SFMM_ERR_ARG, "null argument");
-    *streams = batch->nstream;
+    *streams = batch->own ? static_cast<int>(batch->items.size()) : batch->nstream;
     return SFMM_OK;
 }
 
