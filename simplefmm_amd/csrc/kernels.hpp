@@ -122,8 +122,6 @@ void launch_m2l(const DevPlan& d, int nc, const double* M, double* L, hipStream_
 void launch_l2l(const DevPlan& d, LevelSpan children, int nc, double* L, hipStream_t s);
 void launch_l2p(const DevPlan& d, int nc, const double* L, double* out, hipStream_t s);
 void launch_m2l_mfma(const M2LArgs& a, int p, int nc, const double* M, double* L, hipStream_t s);
-int m2l_rot_nfrag(int p);
-int m2l_coax_nfrag(int p);
 // upper = false: M2M (parents written); true: L2L (children added to).
 void launch_translate(const TransArgs& a, LevelSpan blocks, int p, int nc, bool upper, double* io, hipStream_t s);
 int translate_nfrag(int p, bool upper);

@@ -3,9 +3,6 @@
 
 namespace sfmm {
 
-int m2l_rot_nfrag(int p) { return rot_nfrag(p); }
-int m2l_coax_nfrag(int p) { return coax_nfrag(p); }
-
 void launch_m2l_mfma(const M2LArgs& a, int p, int nc, const double* M, double* L, hipStream_t s)
 {
     if (a.n_block == 0) return;

@@ -8,9 +8,9 @@
 //   stage 2  Z = C Y   per order k: rows j=k..p, cols n=k..p (cos and sin columns
 //                      share the block)
 //   stage 3  L = Q Z   per degree j, same block shapes as stage 1
-// A block is cut into 16-row m-tiles and 4-column k-steps; one "fragment" is the
-// 16x4 A operand of one MFMA, stored as 64 values in lane order
-// (lane l holds A[l % 16][l / 16]).
+// The blocks of each stage are packed block-diagonally into 16x4 MFMA tiles
+// (pack_stage, pack_stage3 below); one "fragment" is the 16x4 A operand of one
+// MFMA, stored as 64 values in lane order (lane l holds A[l % 16][l / 16]).
 //
 // Column-stacked buffers live in LDS in two row orders:
 //   ps(n, b, part): degree-major, per degree the n+1 cos rows then the n sin rows
@@ -32,29 +32,17 @@ __host__ __device__ constexpr int om_row(int P, int k, int n, int part)
     return om_base(P, k) + (part == 0 ? 0 : (P + 1 - k)) + (n - k);
 }
 
-// rotation blocks (stages 1 and 3): k-steps of one (degree, part) tile
+// Fragments per class and direction of one tile set per rotation block (16-row
+// m-tiles, 4-column k-steps): the stride of the rotation fragment table, which
+// bounds every packed layout.
 __host__ __device__ constexpr int rot_nks(int n, int part) { return part == 0 ? ceil_div(n + 1, 4) : ceil_div(n, 4); }
 __host__ __device__ constexpr int rot_mts(int n) { return ceil_div(n + 1, 16); }
-// fragment index of (n, part, mt, ks=0) inside one direction of one polar class
-__host__ __device__ constexpr int rot_frag(int n, int part, int mt)
+__host__ __device__ constexpr int rot_nfrag(int P)
 {
     int f = 0;
-    for (int q = 0; q < n; ++q) f += rot_mts(q) * (rot_nks(q, 0) + rot_nks(q, 1));
-    if (part == 1) f += rot_mts(n) * rot_nks(n, 0);
-    return f + mt * rot_nks(n, part);
+    for (int n = 0; n <= P; ++n) f += rot_mts(n) * (rot_nks(n, 0) + rot_nks(n, 1));
+    return f;
 }
-__host__ __device__ constexpr int rot_nfrag(int P) { return rot_frag(P + 1, 0, 0); }
-
-// coaxial blocks (stage 2)
-__host__ __device__ constexpr int coax_nks(int P, int k) { return ceil_div(P + 1 - k, 4); }
-__host__ __device__ constexpr int coax_mts(int P, int k) { return ceil_div(P + 1 - k, 16); }
-__host__ __device__ constexpr int coax_frag(int P, int k, int mt)
-{
-    int f = 0;
-    for (int q = 0; q < k; ++q) f += coax_mts(P, q) * coax_nks(P, q);
-    return f + mt * coax_nks(P, k);
-}
-__host__ __device__ constexpr int coax_nfrag(int P) { return coax_frag(P, P + 1, 0); }
 
 // Packed tiles of stages 1 and 2.  The blocks of a stage (stage 1: cos block
 // of degree n, rows a = 0..n, cols b = 0..n, and sin block, rows a = 1..n,
