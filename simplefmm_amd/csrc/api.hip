@@ -454,12 +454,16 @@ int run_on(sfmm_plan_s* P, sfmm::Mode mode, int nc, int qdim, const double* dQ, 
     sfmm::launch_fill(P->obuf, static_cast<int64_t>(nc) * d.n, 0.0, s);
     REC(1);
     const bool far = (which & SFMM_FAR) && depth >= 2;  // fmm.hpp:816-820
-    if (far) sfmm::launch_p2m(d, mode, nc, P->qbuf, P->M, s);
+    // the mixed M2L reads an FP32 copy of the multipoles, written by P2M and M2M
+    float* M32 = P->m2l_kernel == 0 && P->m2l[nc].fp32 ? P->M32 : nullptr;
+    if (far) sfmm::launch_p2m(d, mode, nc, P->qbuf, P->M, M32, s);
     REC(2);
+    sfmm::TransArgs m2m = P->m2m[nc];
+    m2m.out32 = M32;
     if (far)
         for (int l = depth - 1; l >= 2; --l) {
             if (P->m2l_kernel == 0)
-                sfmm::launch_translate(P->m2m[nc], level_blocks(P->m2m_lvl[nc], l), P->p, nc, false, P->M, s);
+                sfmm::launch_translate(m2m, level_blocks(P->m2m_lvl[nc], l), P->p, nc, false, P->M, s);
             else
                 sfmm::launch_m2m(d, {P->level_first[l], P->level_first[l + 1] - P->level_first[l]}, nc, P->M, s);
         }
@@ -467,10 +471,7 @@ int run_on(sfmm_plan_s* P, sfmm::Mode mode, int nc, int qdim, const double* dQ, 
     if (far) {
         if (P->m2l_kernel == 0) {
             sfmm::M2LArgs& A = P->m2l[nc];
-            if (A.fp32) {
-                sfmm::launch_to_f32(P->M, P->M32, P->level_first.back() * nc * P->K, s);
-                A.M32 = P->M32;
-            }
+            if (A.fp32) A.M32 = P->M32;
             sfmm::launch_m2l_mfma(A, P->p, nc, P->M, P->L, s);
         }
         else sfmm::launch_m2l(d, nc, P->M, P->L, s);

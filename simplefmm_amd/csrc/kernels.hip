@@ -70,7 +70,7 @@ __global__ void fill_kernel(double* x, int64_t count, double v)
 //    dz         R(n,m) =  sqrt((n-m)(n+m))   R(n-1,m)
 template <int P, int NCQ>
 __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int nc, const double* __restrict__ q,
-                                                 double* __restrict__ M)
+                                                 double* __restrict__ M, float* __restrict__ M32)
 {
     constexpr int K = (P + 1) * (P + 1);
     constexpr int NT = (P + 1) * (P + 2) / 2;  // complex coefficients with m >= 0
@@ -155,6 +155,11 @@ __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int nc, co
     __syncthreads();
 
     double* Mg = M + static_cast<int64_t>(gnode) * nc * K;
+    float* Mg32 = M32 ? M32 + static_cast<int64_t>(gnode) * nc * K : nullptr;  // FP32 copy for the mixed M2L
+    auto put = [&](int i, double v) {
+        Mg[i] = v;
+        if (Mg32) Mg32[i] = static_cast<float>(v);
+    };
     for (int t = tid; t < NT; t += 64) {
         // decode (n, mm) of triangle index t
         int n = 0;
@@ -162,8 +167,8 @@ __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int nc, co
         const int mm = t - n * (n + 1) / 2;
         if (mode == MODE_POT) {
             for (int c = 0; c < nc; ++c) {
-                Mg[c * K + rfc(n, mm)] = S[c][2 * t];
-                if (mm > 0) Mg[c * K + rfs(n, mm)] = S[c][2 * t + 1];
+                put(c * K + rfc(n, mm), S[c][2 * t]);
+                if (mm > 0) put(c * K + rfs(n, mm), S[c][2 * t + 1]);
             }
             continue;
         }
@@ -220,8 +225,8 @@ __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int nc, co
             nout = 3;
         }
         for (int c = 0; c < nout; ++c) {
-            Mg[c * K + rfc(n, mm)] = out[c][0];
-            if (mm > 0) Mg[c * K + rfs(n, mm)] = out[c][1];
+            put(c * K + rfc(n, mm), out[c][0]);
+            if (mm > 0) put(c * K + rfs(n, mm), out[c][1]);
         }
     }
 }
@@ -734,14 +739,15 @@ void with_p(int p, A&&... args)
 
 template <int P>
 struct P2MRun {
-    static void run(const DevPlan& d, int mode, int ncq, int nc, const double* q, double* M, hipStream_t s)
+    static void run(const DevPlan& d, int mode, int ncq, int nc, const double* q, double* M, float* M32,
+                    hipStream_t s)
     {
         const dim3 grid(static_cast<unsigned>(d.n_far_leaf)), blk(64);
         switch (ncq) {
-        case 1: hipLaunchKernelGGL((p2m_kernel<P, 1>), grid, blk, 0, s, d, mode, nc, q, M); break;
-        case 2: hipLaunchKernelGGL((p2m_kernel<P, 2>), grid, blk, 0, s, d, mode, nc, q, M); break;
-        case 3: hipLaunchKernelGGL((p2m_kernel<P, 3>), grid, blk, 0, s, d, mode, nc, q, M); break;
-        default: hipLaunchKernelGGL((p2m_kernel<P, 4>), grid, blk, 0, s, d, mode, nc, q, M); break;
+        case 1: hipLaunchKernelGGL((p2m_kernel<P, 1>), grid, blk, 0, s, d, mode, nc, q, M, M32); break;
+        case 2: hipLaunchKernelGGL((p2m_kernel<P, 2>), grid, blk, 0, s, d, mode, nc, q, M, M32); break;
+        case 3: hipLaunchKernelGGL((p2m_kernel<P, 3>), grid, blk, 0, s, d, mode, nc, q, M, M32); break;
+        default: hipLaunchKernelGGL((p2m_kernel<P, 4>), grid, blk, 0, s, d, mode, nc, q, M, M32); break;
         }
     }
 };
@@ -777,34 +783,11 @@ void launch_fill(double* x, int64_t count, double v, hipStream_t s)
     if (count > 0) hipLaunchKernelGGL(fill_kernel, dim3(grid_for(count, 256)), dim3(256), 0, s, x, count, v);
 }
 
-// y = (float) x, four elements per thread per step
-__global__ void to_f32_kernel(const double* __restrict__ x, float* __restrict__ y, int64_t count)
-{
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x * 4;
-    for (int64_t i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 4; i < count; i += stride) {
-        if (i + 3 < count) {
-            const double4 v = *reinterpret_cast<const double4*>(x + i);
-            *reinterpret_cast<float4*>(y + i) = make_float4(static_cast<float>(v.x), static_cast<float>(v.y),
-                                                            static_cast<float>(v.z), static_cast<float>(v.w));
-        } else {
-            for (int64_t k = i; k < count; ++k) y[k] = static_cast<float>(x[k]);
-        }
-    }
-}
-
-void launch_to_f32(const double* x, float* y, int64_t count, hipStream_t s)
-{
-    if (count <= 0) return;
-    const int64_t blocks = (count + 4 * 256 - 1) / (4 * 256);
-    hipLaunchKernelGGL(to_f32_kernel, dim3(static_cast<unsigned>(blocks < 65536 ? blocks : 65536)), dim3(256), 0, s,
-                       x, y, count);
-}
-
-void launch_p2m(const DevPlan& d, Mode mode, int nc, const double* q, double* M, hipStream_t s)
+void launch_p2m(const DevPlan& d, Mode mode, int nc, const double* q, double* M, float* M32, hipStream_t s)
 {
     if (d.n_far_leaf == 0) return;
     const int ncq = mode == MODE_POT ? nc : (mode == MODE_CURL ? 3 : 1);
-    with_p<P2MRun>(d.p, d, static_cast<int>(mode), ncq, nc, q, M, s);
+    with_p<P2MRun>(d.p, d, static_cast<int>(mode), ncq, nc, q, M, M32, s);
 }
 
 void launch_l2p(const DevPlan& d, int nc, const double* L, double* out, hipStream_t s)

@@ -111,12 +111,14 @@ struct TransArgs {
     const int32_t* ent_oct = nullptr;  // child octant per entry
     const int32_t* ent_in = nullptr;   // [entries][16/nc] input gnodes, -1 = none
     const double* frag = nullptr;      // [8][nfrag][64]
+    float* out32 = nullptr;            // M2M (optional): FP32 copy of the parents written
 };
 
 // All launches are asynchronous on `stream`.
 void launch_gather_charges(const DevPlan& d, int qdim, const double* Q, double* q, hipStream_t s);
 void launch_scatter_outputs(const DevPlan& d, int nout, const double* out, double* U, hipStream_t s);
-void launch_p2m(const DevPlan& d, Mode mode, int nc, const double* q, double* M, hipStream_t s);
+// M32 (optional): an FP32 copy of the leaf multipoles, written alongside M
+void launch_p2m(const DevPlan& d, Mode mode, int nc, const double* q, double* M, float* M32, hipStream_t s);
 void launch_m2m(const DevPlan& d, LevelSpan parents, int nc, double* M, hipStream_t s);
 void launch_m2l(const DevPlan& d, int nc, const double* M, double* L, hipStream_t s);
 void launch_l2l(const DevPlan& d, LevelSpan children, int nc, double* L, hipStream_t s);
@@ -138,7 +140,6 @@ void launch_self_correction(const DevPlan& d, const SelfCorr& c, Mode mode, int 
 void launch_direct(const DevPlan& d, Mode mode, int nc, const double* q, int64_t ntarget, const int64_t* targets,
                    double* out, double* partial, int nsplit, hipStream_t s);
 void launch_fill(double* x, int64_t count, double v, hipStream_t s);
-void launch_to_f32(const double* x, float* y, int64_t count, hipStream_t s);
 
 // Number of double slots the direct sum needs as scratch for `ntarget` targets.
 int64_t direct_scratch_size(int64_t ntarget, int nout, int nsplit);

@@ -160,8 +160,12 @@ __device__ __forceinline__ void translate_wave(const TransArgs& a, int64_t blk, 
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * mt + 4 * r + (lane >> 4);  // f64 16x16x4 D layout
             if (row < K) {
-                if constexpr (UPPER) out[row] += acc[q][r];
-                else out[row] = acc[q][r];
+                if constexpr (UPPER) {
+                    out[row] += acc[q][r];
+                } else {
+                    out[row] = acc[q][r];
+                    if (a.out32) a.out32[(static_cast<int64_t>(dst) * NC + c) * K + row] = static_cast<float>(acc[q][r]);
+                }
             }
         }
     });
