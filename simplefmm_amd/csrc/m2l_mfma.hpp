@@ -381,7 +381,6 @@ __device__ __forceinline__ void stage_rot_fwd(const T* a1, const T* b, T* Y, int
     using F = Mfma<T>;
     using V4 = typename F::V4;
     constexpr auto S = schedule<P, SC>();
-    constexpr int K = (P + 1) * (P + 1);
     constexpr int NQ = S.s1.count[W];
     const int col = lane & 15;
     V4 d[NQ > 0 ? NQ : 1];
@@ -418,7 +417,6 @@ __device__ __forceinline__ void stage_coax(const T* a2, const T* b, T* X, int la
     using F = Mfma<T>;
     using V4 = typename F::V4;
     constexpr auto S = schedule<P, SC>();
-    constexpr int K = (P + 1) * (P + 1);
     constexpr int NQ = S.s2.count[W];
     const int col = lane & 15;
     V4 d[NQ > 0 ? NQ : 1];
