@@ -610,7 +610,13 @@ int sfmm_tree_create(const sfmm_tree_desc* desc, sfmm_tree* out)
         desc->on_device
             ? sfmm::build_tree_device(T->t, desc->n, T->pos.data(), box, desc->max_level, desc->max_per_leaf)
             : sfmm::build_tree(T->t, desc->n, T->pos.data(), box, desc->max_level, desc->max_per_leaf, threads);
-    if (!err.empty()) return fail(SFMM_ERR_ARG, err);
+    if (!err.empty()) {
+        // the device build reports HIP failures as "<call>: <HIP error>"
+        const int code = err.find("out of device memory") != std::string::npos ? SFMM_ERR_ALLOC
+                         : err.rfind("hip", 0) == 0                            ? SFMM_ERR_HIP
+                                                                                : SFMM_ERR_ARG;
+        return fail(code, err);
+    }
     *out = T.release();
     return SFMM_OK;
 }

@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 from simplefmm_amd import api
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -34,3 +36,11 @@ def test_binding_covers_header():
 def test_last_error_is_a_string():
     L = api.lib()
     assert isinstance(L.sfmm_last_error(), bytes)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU driver is present")
+def test_device_tree_build_fails_loudly_without_a_gpu():
+    from simplefmm_amd import workloads as W
+    pos = W.uniform_cube(1000)[0]
+    with pytest.raises(api.SfmmError, match="sfmm error 2"):
+        api.OctreePartitioner(pos, api.Box(pos)).refine(20, 16, on_device=True)
