@@ -45,7 +45,7 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def m2l_flops_survey(p: int) -> int:
+def m2l_flops_complex(p: int) -> int:
     """Per (pair, component) flops of rotate-translate-rotate M2L on complex
     full-m expansions (8 n_rot + 5 n_coax + 12 (p+1)^2)."""
     n_rot = (p + 1) * (2 * p + 1) * (2 * p + 3) // 3
@@ -298,9 +298,9 @@ def main() -> int:
         log(f"[bench] {prec}: rel L2 vs direct on {len(tg)} targets: phi {err_phi:.3e} grad {err_grad:.3e}")
         m2l_ms = mean["M2L"]
         peak = PEAK_FP32 if prec == "mixed" else PEAK_FP64
-        flops_survey = info.m2l_pairs * 4 * m2l_flops_survey(p)
+        flops_complex = info.m2l_pairs * 4 * m2l_flops_complex(p)
         flops_real = info.m2l_pairs * 4 * m2l_flops_real(p)
-        achieved = flops_survey / (m2l_ms * 1e-3)
+        achieved = flops_complex / (m2l_ms * 1e-3)
         del fmm
         return {
             "points_per_sec": n / (ms * 1e-3), "ms_per_step": ms, "plan_build_s": t_b - t_a,
@@ -309,7 +309,7 @@ def main() -> int:
             "stages_ms": mean,
             "roofline": {"kernel": "m2l", "bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                          "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                         "flops_per_launch": flops_survey, "avg_launch_ms": m2l_ms,
+                         "flops_per_launch": flops_complex, "avg_launch_ms": m2l_ms,
                          "executed_flops_per_launch": flops_real,
                          "executed_frac": flops_real / (m2l_ms * 1e-3) / peak},
         }
