@@ -76,6 +76,13 @@ int sfmm_tree_destroy(sfmm_tree tree);
  * the same tree.  No counterpart in the reference; it compares host and device builds. */
 int sfmm_tree_digest(sfmm_tree tree, uint64_t* digest);
 
+/* Near-field sources of original particle i: the particles of the leaves in its
+ * leaf's near list (partition.hpp:324-366), the pairs N2N visits (fmm.hpp:779-785),
+ * as original indices in sorted-slot order; i itself is among them.  *count receives
+ * the number; out (may be NULL) receives up to `capacity` of them, and the call fails
+ * with SFMM_ERR_ARG when capacity is smaller than the count. */
+int sfmm_tree_near_sources(sfmm_tree tree, int64_t i, int64_t* out, int64_t capacity, int64_t* count);
+
 typedef struct {
     int depth;             /* partitioner.level()            */
     int64_t n_node;        /* over all levels                */
@@ -119,6 +126,11 @@ int sfmm_evaluate(sfmm_plan plan, int kind, int ncharge, const double* Q, double
 /* Same with device arrays, asynchronous on the plan's stream (sfmm_plan_stream).
  * dQ: double[qdim*n] AoS original order; dU: double[nout*n], accumulated. */
 int sfmm_evaluate_device(sfmm_plan plan, int kind, int ncharge, const double* dQ, double* dU, int which);
+
+/* FMM3D::slist() (fmm.hpp:398-408,439-442): per original particle i, the entries of
+ * self_generator(i) that lie outside i's near field (subtracted from every near-field
+ * sum), ascending, duplicates kept.  row: int64[n+1] or NULL; val: int64[*nnz] or NULL. */
+int sfmm_plan_slist(sfmm_plan plan, int64_t* row, int64_t* val, int64_t* nnz);
 
 /* The plan's HIP stream (hipStream_t) for sfmm_evaluate_device. */
 int sfmm_plan_stream(sfmm_plan plan, void** stream);

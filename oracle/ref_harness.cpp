@@ -7,15 +7,22 @@
 // Call sequence mirrors main.cpp:38,46-47,54,61,66 (Box -> OctreePartitioner ->
 // refine -> FMM3D<double,3,p,Vector,true> -> rinv / rinv<true>).
 //
-// usage: ref_harness IN OUT P CAP MAXLEVEL MODE [TREEOUT] [REPEAT]
+// usage: ref_harness IN OUT P CAP MAXLEVEL MODE [TREEOUT] [REPEAT] [SELFLIST]
 //   MODE = potgrad : IN holds scalar charges q; writes phi and grad(phi) using
 //                    the curl identity (two rinv<true> runs, Q=(q,0,0), (0,q,0))
 //          vector  : IN holds 3-vector charges; writes rinv (3 potentials) and
 //                    rinv<true> (curl field), as main.cpp does
+//          potential: IN holds scalar charges; writes phi only, from an FMM3D
+//                    built without gradient tables (support_gradient = false,
+//                    fmm.hpp:134,151-153), which fits larger N
 //          time    : like potgrad but prints wall times only (ctor, rinv, rinv<true>)
 //   Every evaluation is written three times: full (rinv), far (rinv_nonear) and
-//   near (N2N with is_self = i==j), unless MODE ends in "_full".
+//   near (N2N), unless MODE ends in "_full".
 //   TREEOUT (or "-") receives the per-level octree lists.
+//   SELFLIST (optional): 'SFMS', int64 n, int64 nnz, int64 row[n+1], int64 val[nnz];
+//   FMM3D is then built with self_generator(i) = val[row[i]..row[i+1]) (fmm.hpp:204)
+//   and evaluated with is_self(i, j) = j in that list (rinv(Q, U, is_self),
+//   fmm.hpp:864-873); without it is_self(i, j) = (i == j) (fmm.hpp:875-881).
 #include "box.hpp"
 #include "fmm.hpp"
 #include "matrix.hpp"
@@ -25,6 +32,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -67,6 +75,36 @@ bool read_input(const char* path, Input& in)
         for (int64_t i = 0; i < in.n; ++i) in.pos[i] = V3{{buf[3 * i], buf[3 * i + 1], buf[3 * i + 2]}};
         in.q.resize(in.qdim * in.n);
         ok = ok && std::fread(in.q.data(), 8, in.q.size(), f) == in.q.size();
+    }
+    std::fclose(f);
+    return ok;
+}
+
+struct SelfList {
+    std::vector<int64_t> row, val;
+    bool given() const { return !row.empty(); }
+    std::vector<int> of(int i) const { return std::vector<int>(val.begin() + row[i], val.begin() + row[i + 1]); }
+    bool has(int i, int j) const
+    {
+        for (int64_t k = row[i]; k < row[i + 1]; ++k)
+            if (val[k] == j) return true;
+        return false;
+    }
+};
+
+bool read_self_list(const char* path, int64_t n, SelfList& s)
+{
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return false;
+    char magic[4];
+    int64_t nn = 0, nnz = 0;
+    bool ok = std::fread(magic, 1, 4, f) == 4 && std::memcmp(magic, "SFMS", 4) == 0 &&
+              std::fread(&nn, 8, 1, f) == 1 && std::fread(&nnz, 8, 1, f) == 1 && nn == n && nnz >= 0;
+    if (ok) {
+        s.row.resize(n + 1);
+        s.val.resize(nnz);
+        ok = std::fread(s.row.data(), 8, n + 1, f) == static_cast<size_t>(n + 1) &&
+             std::fread(s.val.data(), 8, nnz, f) == static_cast<size_t>(nnz) && s.row[0] == 0 && s.row[n] == nnz;
     }
     std::fclose(f);
     return ok;
@@ -170,9 +208,9 @@ void dump_tree(const char* path, const Part& part)
     std::fclose(f);
 }
 
-template <int P>
+template <int P, bool G>
 int run(const Input& in, const char* out_path, int cap, int max_level, const std::string& mode,
-        const char* tree_path, int repeat)
+        const char* tree_path, int repeat, const SelfList& self)
 {
     const int64_t n = in.n;
     const bool full_only = mode.size() > 5 && mode.substr(mode.size() - 5) == "_full";
@@ -186,13 +224,17 @@ int run(const Input& in, const char* out_path, int cap, int max_level, const std
     if (tree_path && std::strcmp(tree_path, "-") != 0) dump_tree(tree_path, part);
     MPI_Barrier(MPI_COMM_WORLD);
     double t2 = now();
-    FMM3D<double, 3, P, Vector, true> fmm(part, MPI_COMM_WORLD);
+    using Fmm = FMM3D<double, 3, P, Vector, G>;
+    std::unique_ptr<Fmm> fmm_ptr =
+        self.given() ? std::make_unique<Fmm>(part, MPI_COMM_WORLD, [&self](int i) { return self.of(i); })
+                     : std::make_unique<Fmm>(part, MPI_COMM_WORLD);
+    const Fmm& fmm = *fmm_ptr;
     MPI_Barrier(MPI_COMM_WORLD);
     double t3 = now();
 
     int rank = 0;
     MPI_Comm_rank(MPI_COMM_WORLD, &rank);
-    const auto is_self = [](int i, int j) { return i == j; };
+    const auto is_self = [&self](int i, int j) { return self.given() ? self.has(i, j) : i == j; };
 
     auto allreduce = [&](std::vector<V3>& U) {
         MPI_Allreduce(MPI_IN_PLACE, U.data(), 3 * static_cast<int>(n), MPI_DOUBLE, MPI_SUM, MPI_COMM_WORLD);
@@ -201,21 +243,28 @@ int run(const Input& in, const char* out_path, int cap, int max_level, const std
     auto eval = [&](const std::vector<V3>& Q, bool grad, int which) {
         std::vector<V3> U(n);
         for (auto& u : U) u = V3{{0.0, 0.0, 0.0}};
-        if (which == 0) {
-            if (grad) fmm.template rinv<true>(Q.data(), U.data());
-            else fmm.template rinv<false>(Q.data(), U.data());
-        } else if (which == 1) {
-            if (grad) fmm.template rinv_nonear<true>(Q.data(), U.data());
-            else fmm.template rinv_nonear<false>(Q.data(), U.data());
+        if constexpr (G) {
+            if (which == 0) {
+                if (grad) fmm.template rinv<true>(Q.data(), U.data(), is_self);
+                else fmm.template rinv<false>(Q.data(), U.data(), is_self);
+            } else if (which == 1) {
+                if (grad) fmm.template rinv_nonear<true>(Q.data(), U.data());
+                else fmm.template rinv_nonear<false>(Q.data(), U.data());
+            } else {
+                if (grad) fmm.template N2N<true>(Q.data(), U.data(), is_self);
+                else fmm.template N2N<false>(Q.data(), U.data(), is_self);
+            }
         } else {
-            if (grad) fmm.template N2N<true>(Q.data(), U.data(), is_self);
+            (void)grad;
+            if (which == 0) fmm.template rinv<false>(Q.data(), U.data(), is_self);
+            else if (which == 1) fmm.template rinv_nonear<false>(Q.data(), U.data());
             else fmm.template N2N<false>(Q.data(), U.data(), is_self);
         }
         allreduce(U);
         return U;
     };
 
-    if (base == "time") {
+    if (base == "time" && G) {
         std::vector<V3> Q1(n), Q2(n);
         for (int64_t i = 0; i < n; ++i) {
             Q1[i] = V3{{in.q[i], 0.0, 0.0}};
@@ -255,7 +304,18 @@ int run(const Input& in, const char* out_path, int cap, int max_level, const std
     }
     const int nwhich = full_only ? 1 : 3;
     const char* suffix[3] = {"full", "far", "near"};
-    if (base == "potgrad") {
+    if (base == "potential") {
+        std::vector<V3> Q1(n);
+        for (int64_t i = 0; i < n; ++i) Q1[i] = V3{{in.q[i], 0.0, 0.0}};
+        for (int w = 0; w < nwhich; ++w) {
+            auto U = eval(Q1, false, w);
+            if (f) {
+                std::vector<double> phi(n);
+                for (int64_t i = 0; i < n; ++i) phi[i] = U[i][0];
+                put_block(f, (std::string("phi_") + suffix[w]).c_str(), phi.data(), n);
+            }
+        }
+    } else if (base == "potgrad" && G) {
         std::vector<V3> Q1(n), Q2(n);
         for (int64_t i = 0; i < n; ++i) {
             Q1[i] = V3{{in.q[i], 0.0, 0.0}};
@@ -280,7 +340,7 @@ int run(const Input& in, const char* out_path, int cap, int max_level, const std
                 put_block(f, (std::string("gzdup_") + suffix[w]).c_str(), redundant.data(), n);
             }
         }
-    } else if (base == "vector") {
+    } else if (base == "vector" && G) {
         std::vector<V3> Q(n);
         for (int64_t i = 0; i < n; ++i) Q[i] = V3{{in.q[3 * i], in.q[3 * i + 1], in.q[3 * i + 2]}};
         for (int w = 0; w < nwhich; ++w) {
@@ -320,15 +380,27 @@ int main(int argc, char** argv)
             const std::string mode = argv[6];
             const char* tree = argc > 7 ? argv[7] : nullptr;
             const int repeat = argc > 8 ? std::atoi(argv[8]) : 1;
-            switch (p) {
-            case 4: rc = run<4>(in, argv[2], cap, max_level, mode, tree, repeat); break;
-            case 6: rc = run<6>(in, argv[2], cap, max_level, mode, tree, repeat); break;
-            case 8: rc = run<8>(in, argv[2], cap, max_level, mode, tree, repeat); break;
-            case 10: rc = run<10>(in, argv[2], cap, max_level, mode, tree, repeat); break;
-            case 12: rc = run<12>(in, argv[2], cap, max_level, mode, tree, repeat); break;
-            case 16: rc = run<16>(in, argv[2], cap, max_level, mode, tree, repeat); break;
-            default: std::fprintf(stderr, "unsupported p=%d\n", p); rc = 2;
+            SelfList self;
+            if (argc > 9 && !read_self_list(argv[9], in.n, self)) {
+                std::fprintf(stderr, "cannot read self list %s\n", argv[9]);
+                rc = 2;
             }
+            // the potential-only modes build FMM3D without gradient tables
+            const bool g = mode.rfind("potential", 0) != 0;
+#define RUN(PP) rc = g ? run<PP, true>(in, argv[2], cap, max_level, mode, tree, repeat, self) \
+                       : run<PP, false>(in, argv[2], cap, max_level, mode, tree, repeat, self)
+            if (rc == 0) {
+                switch (p) {
+                case 4: RUN(4); break;
+                case 6: RUN(6); break;
+                case 8: RUN(8); break;
+                case 10: RUN(10); break;
+                case 12: RUN(12); break;
+                case 16: RUN(16); break;
+                default: std::fprintf(stderr, "unsupported p=%d\n", p); rc = 2;
+                }
+            }
+#undef RUN
         }
     }
     MPI_Finalize();

@@ -71,6 +71,8 @@ SIGNATURES = {
     "sfmm_tree_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TreeInfo)]),
     "sfmm_tree_level_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip]),
     "sfmm_tree_level_lists": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_ip] * 11),
+    "sfmm_tree_near_sources": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _ip, ctypes.c_int64, _ip]),
+    "sfmm_plan_slist": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip, _ip]),
     "sfmm_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PlanDesc), ctypes.POINTER(ctypes.c_void_p)]),
     "sfmm_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sfmm_evaluate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int]),
@@ -200,6 +202,14 @@ class OctreePartitioner:
         _check(lib().sfmm_tree_digest(self.h, ctypes.byref(d)))
         return int(d.value)
 
+    def near_sources(self, i: int) -> np.ndarray:
+        """Original indices of the near-field sources of particle i (its leaf's near list)."""
+        cnt = np.zeros(1, np.int64)
+        _check(lib().sfmm_tree_near_sources(self.h, int(i), None, 0, _i(cnt)))
+        out = np.zeros(int(cnt[0]), np.int64)
+        _check(lib().sfmm_tree_near_sources(self.h, int(i), _i(out), len(out), _i(cnt)))
+        return out
+
     def level_lists(self, l: int) -> dict:
         c = np.zeros(6, dtype=np.int64)
         _check(lib().sfmm_tree_level_counts(self.h, l, _i(c)))
@@ -293,6 +303,15 @@ class FMM3D:
 
     def synchronize(self):
         _check(lib().sfmm_synchronize(self.h))
+
+    def slist(self):
+        """The far self pairs (_slist, fmm.hpp:398-408): (row, val) CSR over original indices."""
+        nnz = np.zeros(1, np.int64)
+        _check(lib().sfmm_plan_slist(self.h, None, None, _i(nnz)))
+        row = np.zeros(self.n + 1, np.int64)
+        val = np.zeros(max(int(nnz[0]), 1), np.int64)
+        _check(lib().sfmm_plan_slist(self.h, _i(row), _i(val), _i(nnz)))
+        return row, val[: int(nnz[0])]
 
     def stream(self) -> int:
         s = ctypes.c_void_p()

@@ -5,14 +5,24 @@
 // the MI355X through the sfmm C-ABI (include/sfmm.h).
 //
 // Differences a caller can see:
-//   * the communicator argument is accepted and ignored (one process, one GPU);
+//   * the communicator argument is stored and returned by comm() but not used
+//     (one process, one GPU): its type is MPI_Comm when <mpi.h> is included
+//     first, an empty tag otherwise;
 //   * T must be double; p must be in 1..16; gradient (curl) needs N == 3, as in
 //     the reference (fmm.hpp:550);
-//   * rinv(Q, U, is_self) and N2N(Q, U, is_self) take is_self(i, j) to be
-//     "j is in self_generator(i)" (the generator given at construction; by
-//     default i -> {i}); is_self itself is not evaluated.  rinv(Q, U) excludes
-//     only j == i in the near field, as in the reference; far-field pairs of
-//     the generator are subtracted in both (_slist);
+//   * rinv(Q, U, is_self) and N2N(Q, U, is_self) with the predicate returned by
+//     self_list() (j in self_generator(i)) exclude the listed pairs on the GPU.
+//     Any other callable is evaluated on the host, as the reference evaluates it,
+//     for every near-field pair (i, j != i): the GPU sums the near field without
+//     those exclusions and the pairs with is_self(i, j) true are then subtracted
+//     on the host in FP64.  This is exact but costs one host call per near pair.
+//     The pair (i, i) is always excluded (the reference would divide by zero);
+//   * far-field pairs of the generator (slist(), the reference's _slist) are
+//     subtracted in every near-field sum, as in the reference (fmm.hpp:798-807);
+//   * the per-level stage methods N2M, M2M, M2L, L2L, L2N (fmm.hpp:524-771) are
+//     not exposed: the expansions live on the device in a level-scaled real form,
+//     not in the reference's LevelData; rinv_nonear (far field), N2N (near field)
+//     and the per-stage device times (sfmm_last_stage_ms) cover their uses;
 //   * a failing call prints the library's message and aborts, where the
 //     reference would assert.
 #pragma once
@@ -22,12 +32,19 @@
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
+#include <tuple>
 #include <ostream>
 #include <type_traits>
 #include <utility>
 #include <vector>
 
 #include "../../include/sfmm.h"
+
+#ifdef MPI_VERSION
+using sfmm_comm_t = MPI_Comm;
+#else
+struct sfmm_comm_t {};  // no MPI: an opaque tag
+#endif
 
 namespace sfmm_compat {
 
@@ -173,6 +190,20 @@ private:
     sfmm_tree _tree = nullptr;
 };
 
+// Rows of (int) values, the read interface of the reference's CSRP<> (csr.hpp:58-103).
+class CSRList {
+public:
+    CSRList() = default;
+    CSRList(std::vector<int64_t> row, std::vector<int64_t> val) : _row(std::move(row)), _val(std::move(val)) {}
+    int nrow() const noexcept { return _row.empty() ? 0 : static_cast<int>(_row.size() - 1); }
+    int nnz(int row) const noexcept { return static_cast<int>(_row[row + 1] - _row[row]); }
+    int nnz() const noexcept { return _row.empty() ? 0 : static_cast<int>(_row.back()); }
+    std::tuple<int> value(int row, int k) const noexcept { return {static_cast<int>(_val[_row[row] + k])}; }
+
+private:
+    std::vector<int64_t> _row, _val;
+};
+
 template <typename T, size_t N, int p, template <typename, size_t> typename Container, bool support_gradient = false>
 class FMM3D {
     static_assert(std::is_same_v<T, double>, "the MI355X evaluator computes in FP64");
@@ -182,22 +213,44 @@ class FMM3D {
 public:
     using Partitioner_t = OctreePartitioner<T, Container>;
 
+    using comm_type = sfmm_comm_t;
+
+    // is_self(i, j) = "j is in self_generator(i)", evaluated on the GPU by rinv / N2N
+    class SelfList {
+    public:
+        bool operator()(int i, int j) const noexcept
+        {
+            if (_row->empty()) return i == j;
+            for (int64_t k = (*_row)[i]; k < (*_row)[i + 1]; ++k)
+                if ((*_val)[k] == j) return true;
+            return false;
+        }
+
+    private:
+        friend class FMM3D;
+        SelfList(const std::vector<int64_t>* row, const std::vector<int64_t>* val) : _row(row), _val(val) {}
+        const std::vector<int64_t>* _row;
+        const std::vector<int64_t>* _val;
+    };
+
     template <typename Comm>
-    FMM3D(const Partitioner_t& partitioner, Comm, const std::function<std::vector<int>(int)>& self_generator) :
+    FMM3D(const Partitioner_t& partitioner, Comm comm, const std::function<std::vector<int>(int)>& self_generator) :
         _partitioner(partitioner)
     {
+        keep_comm(comm);
         const int64_t n = static_cast<int64_t>(partitioner.positions().size());
-        std::vector<int64_t> row(1, 0), val;
+        _gen_row.assign(1, 0);
         for (int64_t i = 0; i < n; ++i) {
-            for (int j : self_generator(static_cast<int>(i))) val.push_back(j);
-            row.push_back(static_cast<int64_t>(val.size()));
+            for (int j : self_generator(static_cast<int>(i))) _gen_val.push_back(j);
+            _gen_row.push_back(static_cast<int64_t>(_gen_val.size()));
         }
-        create(row.data(), val.data());
+        create(_gen_row.data(), _gen_val.data());
     }
 
     template <typename Comm>
-    FMM3D(const Partitioner_t& partitioner, Comm) : _partitioner(partitioner)
+    FMM3D(const Partitioner_t& partitioner, Comm comm) : _partitioner(partitioner)
     {
+        keep_comm(comm);
         create(nullptr, nullptr);
     }
 
@@ -209,6 +262,25 @@ public:
     }
 
     const Partitioner_t& partitioner() const noexcept { return _partitioner; }
+    comm_type comm() const noexcept { return _comm; }
+
+    // _slist (fmm.hpp:398-408,439-442): the generator's entries outside each particle's near field
+    const CSRList& slist() const
+    {
+        if (!_slist_ready) {
+            const int64_t n = static_cast<int64_t>(_partitioner.positions().size());
+            int64_t nnz = 0;
+            sfmm_compat::check(sfmm_plan_slist(_plan, nullptr, nullptr, &nnz), "FMM3D::slist");
+            std::vector<int64_t> row(n + 1), val(nnz);
+            sfmm_compat::check(sfmm_plan_slist(_plan, row.data(), val.data(), &nnz), "FMM3D::slist");
+            _slist = CSRList(std::move(row), std::move(val));
+            _slist_ready = true;
+        }
+        return _slist;
+    }
+
+    // The predicate rinv / N2N evaluate on the GPU: membership in self_generator(i).
+    SelfList self_list() const noexcept { return SelfList(&_gen_row, &_gen_val); }
 
     template <bool gradient = false>
     void rinv(const Container<T, N>* Q, Container<T, N>* U) const noexcept
@@ -217,9 +289,14 @@ public:
     }
 
     template <bool gradient = false, typename IsSelf>
-    void rinv(const Container<T, N>* Q, Container<T, N>* U, const IsSelf&) const noexcept
+    void rinv(const Container<T, N>* Q, Container<T, N>* U, const IsSelf& is_self) const noexcept
     {
-        eval<gradient>(Q, U, SFMM_BOTH | SFMM_SELF_LIST);
+        if constexpr (std::is_same_v<IsSelf, SelfList>) {
+            eval<gradient>(Q, U, SFMM_BOTH | SFMM_SELF_LIST);
+        } else {
+            eval<gradient>(Q, U, SFMM_BOTH);
+            subtract_excluded<gradient>(Q, U, is_self);
+        }
     }
 
     template <bool gradient = false>
@@ -229,9 +306,14 @@ public:
     }
 
     template <bool gradient = false, typename IsSelf>
-    void N2N(const Container<T, N>* Q, Container<T, N>* U, const IsSelf&) const noexcept
+    void N2N(const Container<T, N>* Q, Container<T, N>* U, const IsSelf& is_self) const noexcept
     {
-        eval<gradient>(Q, U, SFMM_NEAR | SFMM_SELF_LIST);
+        if constexpr (std::is_same_v<IsSelf, SelfList>) {
+            eval<gradient>(Q, U, SFMM_NEAR | SFMM_SELF_LIST);
+        } else {
+            eval<gradient>(Q, U, SFMM_NEAR);
+            subtract_excluded<gradient>(Q, U, is_self);
+        }
     }
 
     // Scalar charges q[i]: out[4i..4i+3] += phi, dphi/dx, dphi/dy, dphi/dz.
@@ -241,6 +323,43 @@ public:
     }
 
 private:
+    template <typename Comm>
+    void keep_comm(const Comm& comm)
+    {
+        if constexpr (std::is_convertible_v<Comm, comm_type>) _comm = comm;
+    }
+
+    // The near-field pairs (i, j != i) a general is_self excludes, subtracted on the
+    // host in FP64 after the GPU summed them (N2N, fmm.hpp:786-796).
+    template <bool gradient, typename IsSelf>
+    void subtract_excluded(const Container<T, N>* Q, Container<T, N>* U, const IsSelf& is_self) const
+    {
+        const auto& x = _partitioner.positions();
+        const int64_t n = static_cast<int64_t>(x.size());
+        std::vector<int64_t> src;
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t cnt = 0;
+            sfmm_compat::check(sfmm_tree_near_sources(_partitioner.handle(), i, nullptr, 0, &cnt), "FMM3D::N2N");
+            src.resize(static_cast<size_t>(cnt));
+            sfmm_compat::check(sfmm_tree_near_sources(_partitioner.handle(), i, src.data(), cnt, &cnt), "FMM3D::N2N");
+            for (int64_t j : src) {
+                if (j == i || !is_self(static_cast<int>(i), static_cast<int>(j))) continue;
+                T R[3];
+                for (int d = 0; d < 3; ++d) R[d] = x[i][d] - x[j][d];
+                const T r = std::sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+                if constexpr (gradient) {
+                    const T r3 = r * r * r;
+                    const auto& q = Q[j];
+                    U[i][0] -= (q[1] * R[2] - q[2] * R[1]) / r3;
+                    U[i][1] -= (q[2] * R[0] - q[0] * R[2]) / r3;
+                    U[i][2] -= (q[0] * R[1] - q[1] * R[0]) / r3;
+                } else {
+                    for (size_t c = 0; c < N; ++c) U[i][c] -= Q[j][c] / r;
+                }
+            }
+        }
+    }
+
     void create(const int64_t* self_row, const int64_t* self_val)
     {
         sfmm_plan_desc d{};
@@ -282,4 +401,8 @@ private:
 
     const Partitioner_t& _partitioner;
     sfmm_plan _plan = nullptr;
+    comm_type _comm{};
+    std::vector<int64_t> _gen_row, _gen_val;  // self_generator lists (empty: i -> {i})
+    mutable CSRList _slist;
+    mutable bool _slist_ready = false;
 };

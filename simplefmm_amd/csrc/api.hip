@@ -16,9 +16,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -57,6 +59,40 @@ int fail(int code, const std::string& msg)
     return code;
 }
 
+// Every device allocation of the library goes through here.  Fault injection
+// for tests: while the environment variable SFMM_FAULT_ALLOC holds a count k,
+// the first k allocations after it was set (or changed) succeed and every
+// later one fails as if the device were out of memory.
+hipError_t dev_malloc(void** p, size_t bytes)
+{
+    static std::mutex mu;
+    static std::string armed;
+    static long made = 0;
+    if (const char* f = std::getenv("SFMM_FAULT_ALLOC")) {
+        std::lock_guard<std::mutex> lock(mu);
+        if (armed != f) {
+            armed = f;
+            made = 0;
+        }
+        if (made++ >= std::strtol(f, nullptr, 10)) {
+            *p = nullptr;
+            return hipErrorOutOfMemory;
+        }
+    } else {
+        std::lock_guard<std::mutex> lock(mu);
+        armed.clear();
+    }
+    return hipMalloc(p, bytes);
+}
+template <class T>
+hipError_t dev_malloc(T** p, size_t bytes)
+{
+    void* v = nullptr;
+    const hipError_t e = dev_malloc(&v, bytes);
+    *p = static_cast<T*>(v);
+    return e;
+}
+
 #define HIP_TRY(expr)                                                                            \
     do {                                                                                         \
         hipError_t e_ = (expr);                                                                  \
@@ -68,6 +104,23 @@ int fail(int code, const std::string& msg)
 struct sfmm_tree_s {
     sfmm::Tree t;
     std::vector<double> pos;  // copy of the caller's positions
+    // original index -> sorted slot, sorted slot -> global leaf (built on first use)
+    std::vector<int32_t> iperm, slot_leaf;
+    std::once_flag maps_once;
+
+    void build_maps()
+    {
+        std::call_once(maps_once, [this] {
+            iperm.assign(t.n, 0);
+            for (int64_t s = 0; s < t.n; ++s) iperm[t.perm[s]] = static_cast<int32_t>(s);
+            slot_leaf.assign(t.n, -1);
+            for (size_t g = 0; g < t.leaf_node.size(); ++g) {
+                const auto& lv = t.lv[t.leaf_level[g]];
+                const int32_t a = t.leaf_node[g];
+                for (int64_t s = lv.pbegin[a]; s < lv.pend[a]; ++s) slot_leaf[s] = static_cast<int32_t>(g);
+            }
+        });
+    }
 };
 
 struct sfmm_plan_s {
@@ -104,6 +157,7 @@ struct sfmm_plan_s {
     const double* l2l_frag = nullptr;
     sfmm::TransArgs m2m[5], l2l[5];     // per component count (built on first use)
     sfmm::SelfCorr self;                // self-list corrections (empty without a self generator)
+    std::vector<int64_t> slist_row, slist_val;  // far self pairs per particle, original order (_slist)
     std::vector<int64_t> m2m_lvl[5], l2l_lvl[5];  // first block of each level (+ end)
     hipEvent_t* last_ev = nullptr;   // event set of the last evaluation
     std::vector<hipEvent_t> rec_ev;  // sets of 9 events, one set per recorded evaluation
@@ -132,7 +186,7 @@ struct sfmm_plan_s {
         *out = nullptr;
         if (v.empty()) return SFMM_OK;
         void* p = nullptr;
-        if (hipMalloc(&p, v.size() * sizeof(T)) != hipSuccess) return fail(SFMM_ERR_ALLOC, "hipMalloc failed (plan tables)");
+        if (dev_malloc(&p, v.size() * sizeof(T)) != hipSuccess) return fail(SFMM_ERR_ALLOC, "hipMalloc failed (plan tables)");
         owned.push_back(p);
         HIP_TRY(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
         *out = static_cast<const T*>(p);
@@ -149,13 +203,16 @@ struct sfmm_batch_s {
     std::vector<hipEvent_t> fork, join;
     int nstream = 1;
     bool own = false;               // every item on its plan's own stream
-    std::vector<hipEvent_t> pjoin;  // per item, for `own`
+    std::vector<hipEvent_t> pjoin;      // per item, for `own`
+    std::vector<hipEvent_t> pre, post;  // per item: its plan's stream before / after the item
 
     ~sfmm_batch_s()
     {
         for (auto e : fork) (void)hipEventDestroy(e);
         for (auto e : join) (void)hipEventDestroy(e);
         for (auto e : pjoin) (void)hipEventDestroy(e);
+        for (auto e : pre) (void)hipEventDestroy(e);
+        for (auto e : post) (void)hipEventDestroy(e);
         for (auto st : streams) (void)hipStreamDestroy(st);
     }
 };
@@ -197,27 +254,27 @@ int ensure_buffers(sfmm_plan_s* P, int nc, int qdim)
         P->M32 = nullptr;
         P->nc_alloc = 0;
         const size_t bytes = sizeof(double) * static_cast<size_t>(nnode) * nc * P->K;
-        if (hipMalloc(&P->M, bytes) != hipSuccess || hipMalloc(&P->L, bytes) != hipSuccess)
+        if (dev_malloc(&P->M, bytes) != hipSuccess || dev_malloc(&P->L, bytes) != hipSuccess)
             return fail(SFMM_ERR_ALLOC, "hipMalloc failed (expansions)");
-        if (P->precision == SFMM_MIXED && P->m2l_kernel == 0 && hipMalloc(&P->M32, bytes / 2) != hipSuccess)
+        if (P->precision == SFMM_MIXED && P->m2l_kernel == 0 && dev_malloc(&P->M32, bytes / 2) != hipSuccess)
             return fail(SFMM_ERR_ALLOC, "hipMalloc failed (FP32 multipoles)");
         P->nc_alloc = nc;
     }
     if (qdim > P->qdim_alloc) {
         (void)hipFree(P->qbuf);
         P->qbuf = nullptr;
-        if (hipMalloc(&P->qbuf, sizeof(double) * n * qdim) != hipSuccess)
+        if (dev_malloc(&P->qbuf, sizeof(double) * n * qdim) != hipSuccess)
             return fail(SFMM_ERR_ALLOC, "hipMalloc failed (charges)");
         P->qdim_alloc = qdim;
     }
     if (P->p2p.n_partial_rows > 0 && !P->p2p_partial) {
-        if (hipMalloc(&P->p2p_partial, sizeof(double) * 4 * P->p2p.n_partial_rows) != hipSuccess)
+        if (dev_malloc(&P->p2p_partial, sizeof(double) * 4 * P->p2p.n_partial_rows) != hipSuccess)
             return fail(SFMM_ERR_ALLOC, "hipMalloc failed (near-field partial sums)");
     }
     if (nc > P->nout_alloc) {
         (void)hipFree(P->obuf);
         P->obuf = nullptr;
-        if (hipMalloc(&P->obuf, sizeof(double) * n * nc) != hipSuccess)
+        if (dev_malloc(&P->obuf, sizeof(double) * n * nc) != hipSuccess)
             return fail(SFMM_ERR_ALLOC, "hipMalloc failed (outputs)");
         P->nout_alloc = nc;
     }
@@ -228,8 +285,10 @@ int ensure_buffers(sfmm_plan_s* P, int nc, int qdim)
 // lists, per block, the offsets that occur with the source of every target.
 int build_m2l_blocks(sfmm_plan_s* P, int nc)
 {
-    sfmm::M2LArgs& A = P->m2l[nc];
-    if (A.blk_row) return SFMM_OK;
+    // built into a local and published only when every upload succeeded, so a
+    // failed attempt leaves the plan without blocks and the next one retries
+    if (P->m2l[nc].blk_row) return SFMM_OK;
+    sfmm::M2LArgs A;
     const sfmm::Tree& t = P->tree->t;
     const int TB = 16 / nc;
     std::vector<int32_t> blk_tgt, ent_src;
@@ -318,6 +377,7 @@ int build_m2l_blocks(sfmm_plan_s* P, int nc)
     A.coaxA = A.fp32 ? static_cast<const void*>(P->coaxA32) : static_cast<const void*>(P->coaxA64);
     A.off_cs = P->d.off_cs;
     A.zero = P->zero_cols;
+    P->m2l[nc] = A;
     return SFMM_OK;
 }
 
@@ -392,8 +452,7 @@ int build_translate_blocks(sfmm_plan_s* P, int nc)
     }
     lv_m[t.depth + 1] = static_cast<int64_t>(row.size()) - 1;
     lv_l[t.depth + 1] = static_cast<int64_t>(lrow.size()) - 1;
-    sfmm::TransArgs& A = P->m2m[nc];
-    sfmm::TransArgs& B = P->l2l[nc];
+    sfmm::TransArgs A, B;  // published together once every upload succeeded
     int rc;
     if ((rc = P->upload(row, &A.blk_row)) || (rc = P->upload(out, &A.blk_out)) || (rc = P->upload(oct, &A.ent_oct)) ||
         (rc = P->upload(in, &A.ent_in)) || (rc = P->upload(lrow, &B.blk_row)) || (rc = P->upload(lout, &B.blk_out)) ||
@@ -401,6 +460,8 @@ int build_translate_blocks(sfmm_plan_s* P, int nc)
         return rc;
     A.frag = P->m2m_frag;
     B.frag = P->l2l_frag;
+    P->m2m[nc] = A;
+    P->l2l[nc] = B;
     return SFMM_OK;
 }
 
@@ -550,7 +611,14 @@ int batch_issue(sfmm_batch_s* B, const std::vector<sfmm_batch_item>& items, cons
         sfmm::Mode mode;
         int qdim, nc;
         if (int r = mode_of(it.kind, it.ncharge, mode, qdim, nc)) return r;
-        if (int r = run_on(it.plan, mode, nc, qdim, it.dQ, it.dU, it.which, B->streams[where[i]], false)) return r;
+        // the plan's buffers are shared with its own stream: the item runs after the
+        // work already queued there, and later work there runs after the item
+        hipStream_t bs = B->streams[where[i]];
+        HIP_TRY(hipEventRecord(B->pre[i], it.plan->stream));
+        HIP_TRY(hipStreamWaitEvent(bs, B->pre[i], 0));
+        if (int r = run_on(it.plan, mode, nc, qdim, it.dQ, it.dU, it.which, bs, false)) return r;
+        HIP_TRY(hipEventRecord(B->post[i], bs));
+        HIP_TRY(hipStreamWaitEvent(it.plan->stream, B->post[i], 0));
     }
     for (int k = 0; k < ns; ++k) {
         HIP_TRY(hipEventRecord(B->join[k], B->streams[k]));
@@ -643,6 +711,22 @@ int sfmm_tree_digest(sfmm_tree tree, uint64_t* digest)
     f.vec(t.leaf_level), f.vec(t.leaf_node), f.vec(t.leaf_row), f.vec(t.near_row), f.vec(t.near_begin);
     f.vec(t.near_end);
     *digest = f.h;
+    return SFMM_OK;
+}
+
+int sfmm_tree_near_sources(sfmm_tree tree, int64_t i, int64_t* out, int64_t capacity, int64_t* count)
+{
+    if (!tree || !count) return fail(SFMM_ERR_ARG, "null argument");
+    const sfmm::Tree& t = tree->t;
+    if (i < 0 || i >= t.n) return fail(SFMM_ERR_ARG, "particle index out of range");
+    tree->build_maps();
+    const int32_t g = tree->slot_leaf[tree->iperm[i]];
+    int64_t cnt = 0;
+    for (int64_t r = t.near_row[g]; r < t.near_row[g + 1]; ++r)
+        for (int64_t sl = t.near_begin[r]; sl < t.near_end[r]; ++sl, ++cnt)
+            if (out && cnt < capacity) out[cnt] = t.perm[sl];
+    *count = cnt;
+    if (out && cnt > capacity) return fail(SFMM_ERR_ARG, "capacity is smaller than the number of near sources");
     return SFMM_OK;
 }
 
@@ -804,11 +888,15 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
         std::vector<int32_t> slot_leaf(n);
         for (size_t g = 0; g < lf_begin.size(); ++g)
             for (int64_t s = lf_begin[g]; s < lf_end[g]; ++s) slot_leaf[s] = static_cast<int32_t>(g);
+        P->slist_row.assign(1, 0);
+        std::vector<int64_t> near_seen, far_j;
         for (int64_t i = 0; i < n; ++i) {
             if (row[i + 1] < row[i]) return fail(SFMM_ERR_ARG, "self_row must be non-decreasing");
             const int64_t ti = iperm[i];
             const int32_t g = slot_leaf[ti];
             bool any = false;
+            near_seen.clear();
+            far_j.clear();
             for (int64_t k = row[i]; k < row[i + 1]; ++k) {
                 const int64_t j = val[k];
                 if (j < 0 || j >= n) return fail(SFMM_ERR_ARG, "self_val entry outside [0, n)");
@@ -817,6 +905,15 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
                 bool near = false;
                 for (int64_t r = t.near_row[g]; r < t.near_row[g + 1] && !near; ++r)
                     near = t.near_begin[r] <= sj && sj < t.near_end[r];
+                // a near source listed twice is still excluded from the near sum once
+                // (is_self is a predicate, fmm.hpp:786); a far one listed twice is
+                // subtracted twice (_slist keeps every listed entry, fmm.hpp:398-408)
+                if (near) {
+                    if (std::find(near_seen.begin(), near_seen.end(), sj) != near_seen.end()) continue;
+                    near_seen.push_back(sj);
+                } else {
+                    far_j.push_back(j);
+                }
                 sc_src.push_back(sj);
                 sc_near.push_back(near ? 1 : 0);
                 any = true;
@@ -825,6 +922,9 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
                 sc_tgt.push_back(ti);
                 sc_row.push_back(static_cast<int64_t>(sc_src.size()));
             }
+            std::sort(far_j.begin(), far_j.end());  // _slist rows are sorted (csr.hpp:38-56)
+            P->slist_val.insert(P->slist_val.end(), far_j.begin(), far_j.end());
+            P->slist_row.push_back(static_cast<int64_t>(P->slist_val.size()));
         }
     }
     // ---- M2L work of the row kernel: all nodes at levels >= 2 (the MFMA kernel
@@ -985,6 +1085,21 @@ int sfmm_plan_destroy(sfmm_plan plan)
     return SFMM_OK;
 }
 
+int sfmm_plan_slist(sfmm_plan plan, int64_t* row, int64_t* val, int64_t* nnz)
+{
+    if (!plan || !nnz) return fail(SFMM_ERR_ARG, "null argument");
+    const int64_t n = plan->tree->t.n;
+    const bool given = !plan->slist_row.empty();
+    *nnz = given ? plan->slist_row.back() : 0;
+    if (row) {
+        if (given) std::memcpy(row, plan->slist_row.data(), sizeof(int64_t) * (n + 1));
+        else std::fill(row, row + n + 1, int64_t(0));
+    }
+    if (val && given && !plan->slist_val.empty())
+        std::memcpy(val, plan->slist_val.data(), sizeof(int64_t) * plan->slist_val.size());
+    return SFMM_OK;
+}
+
 int sfmm_plan_stream(sfmm_plan plan, void** stream)
 {
     if (!plan || !stream) return fail(SFMM_ERR_ARG, "null argument");
@@ -1017,13 +1132,13 @@ int sfmm_evaluate(sfmm_plan plan, int kind, int ncharge, const double* Q, double
     if (qdim * n > plan->stage_q) {
         (void)hipFree(plan->hQ);
         plan->hQ = nullptr;
-        if (hipMalloc(&plan->hQ, sizeof(double) * qdim * n) != hipSuccess) return fail(SFMM_ERR_ALLOC, "hipMalloc failed (staging)");
+        if (dev_malloc(&plan->hQ, sizeof(double) * qdim * n) != hipSuccess) return fail(SFMM_ERR_ALLOC, "hipMalloc failed (staging)");
         plan->stage_q = qdim * n;
     }
     if (nc * n > plan->stage_u) {
         (void)hipFree(plan->hU);
         plan->hU = nullptr;
-        if (hipMalloc(&plan->hU, sizeof(double) * nc * n) != hipSuccess) return fail(SFMM_ERR_ALLOC, "hipMalloc failed (staging)");
+        if (dev_malloc(&plan->hU, sizeof(double) * nc * n) != hipSuccess) return fail(SFMM_ERR_ALLOC, "hipMalloc failed (staging)");
         plan->stage_u = nc * n;
     }
     HIP_TRY(hipMemcpyAsync(plan->hQ, Q, sizeof(double) * qdim * n, hipMemcpyHostToDevice, plan->stream));
@@ -1105,10 +1220,10 @@ int sfmm_direct(sfmm_plan plan, int kind, int ncharge, const double* Q, int64_t 
         (void)hipFree(part);
         (void)hipFree(dt);
     };
-    if (hipMalloc(&dQ, sizeof(double) * qdim * n) != hipSuccess ||
-        hipMalloc(&dout, sizeof(double) * nc * ntarget) != hipSuccess ||
-        hipMalloc(&part, sizeof(double) * sfmm::direct_scratch_size(ntarget, nc, nsplit)) != hipSuccess ||
-        hipMalloc(&dt, sizeof(int64_t) * ntarget) != hipSuccess) {
+    if (dev_malloc(&dQ, sizeof(double) * qdim * n) != hipSuccess ||
+        dev_malloc(&dout, sizeof(double) * nc * ntarget) != hipSuccess ||
+        dev_malloc(&part, sizeof(double) * sfmm::direct_scratch_size(ntarget, nc, nsplit)) != hipSuccess ||
+        dev_malloc(&dt, sizeof(int64_t) * ntarget) != hipSuccess) {
         cleanup();
         return fail(SFMM_ERR_ALLOC, "hipMalloc failed (direct sum)");
     }
@@ -1163,6 +1278,10 @@ int sfmm_batch_create(const sfmm_batch_item* items, int count, int streams, sfmm
     for (auto& e : B->join) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     B->pjoin.resize(count);
     for (auto& e : B->pjoin) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    B->pre.resize(count);
+    B->post.resize(count);
+    for (auto& e : B->pre) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : B->post) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
     int ns = std::min(streams, count);
     if (streams == 0) {
@@ -1173,7 +1292,7 @@ int sfmm_batch_create(const sfmm_batch_item* items, int count, int streams, sfmm
         };
         std::vector<sfmm_batch_item> trial = B->items;
         for (int i = 0; i < count; ++i) {
-            if (hipMalloc(&scratch[i], sizeof(double) * nout[i]) != hipSuccess) {
+            if (dev_malloc(&scratch[i], sizeof(double) * nout[i]) != hipSuccess) {
                 release();
                 return fail(SFMM_ERR_ALLOC, "hipMalloc failed (batch tuning outputs)");
             }

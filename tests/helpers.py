@@ -12,14 +12,30 @@ from simplefmm_amd import workloads as W
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def golden_cases(mode: str | None = None):
+def golden_cases(mode: str | None = None, self_lists: bool | None = False, max_n: int | None = None):
+    """Fixture names, optionally by mode, by whether the reference ran with a
+    self generator (None: either), and by size."""
     out = []
     for f in sorted(os.listdir(GOLDEN)):
         if f.endswith(".json"):
             meta = json.load(open(os.path.join(GOLDEN, f)))
-            if mode is None or meta["mode"] == mode:
-                out.append(meta["case"])
+            if mode is not None and meta["mode"] != mode:
+                continue
+            if self_lists is not None and bool(meta.get("self_lists")) != self_lists:
+                continue
+            if max_n is not None and meta["n"] > max_n:
+                continue
+            out.append(meta["case"])
     return out
+
+
+def golden_self_lists(meta, pos):
+    """The self generator lists a self-list fixture was made with."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(GOLDEN, "make_golden.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.self_lists(pos, meta["seed"])
 
 
 def load_golden(case: str):

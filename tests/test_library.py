@@ -44,3 +44,13 @@ def test_device_tree_build_fails_loudly_without_a_gpu():
     pos = W.uniform_cube(1000)[0]
     with pytest.raises(api.SfmmError, match="sfmm error 2"):
         api.OctreePartitioner(pos, api.Box(pos)).refine(20, 16, on_device=True)
+
+
+def test_cpp_dropin_compiles():
+    # the drop-in header (simplefmm_amd/cpp/fmm.hpp) with the calls code written for
+    # the reference makes: self_generator constructor, comm(), slist(), rinv / N2N
+    # with a lambda predicate (tests/dropin_api.cpp; run on the GPU by test_gpu_api.py)
+    import subprocess
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-I",
+                    os.path.join(ROOT, "simplefmm_amd", "cpp"), os.path.join(ROOT, "tests", "dropin_api.cpp")],
+                   check=True, timeout=120)

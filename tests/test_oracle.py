@@ -14,9 +14,9 @@ import pytest
 
 from oracle import oracle as O
 from simplefmm_amd import api, workloads as W
-from tests.helpers import golden_cases, load_golden, rel_l2
+from tests.helpers import golden_cases, golden_self_lists, load_golden, rel_l2
 
-SMALL_POTGRAD = [c for c in golden_cases("potgrad") if load_golden(c)[0]["n"] <= 20000]
+SMALL_POTGRAD = golden_cases("potgrad", max_n=20000)
 
 
 @pytest.mark.parametrize("case", SMALL_POTGRAD)
@@ -42,6 +42,39 @@ def test_oracle_matches_reference_vector():
         curl = t.fmm(meta["p"], Q, O.CURL, which)
         assert rel_l2(pot[idx], g[f"pot_{suffix}"]) < 1e-12
         assert rel_l2(curl[idx], g[f"curl_{suffix}"]) < 1e-12
+
+
+def test_oracle_matches_reference_self_generator():
+    # FMM3D(partitioner, comm, self_generator) + rinv(Q, U, is_self) with is_self =
+    # membership (fmm.hpp:204,398-408,786-807): near pairs excluded once even when
+    # listed twice, far pairs subtracted once per listing
+    meta, g, pos, q = load_golden("selfgen_6k_p6")
+    row, val = golden_self_lists(meta, pos)
+    t = O.Tree(pos, meta["max_level"], meta["cap"])
+    idx = g["idx"]
+    far_listed = sum(1 for i in range(meta["n"]) for j in val[row[i]:row[i + 1]] if not t.is_neighbor(i, j))
+    near_listed = sum(1 for i in range(meta["n"]) for j in val[row[i]:row[i + 1]] if j != i and t.is_neighbor(i, j))
+    assert far_listed > 1000 and near_listed > 1000
+    for which, suffix in ((O.BOTH, "full"), (O.FAR, "far"), (O.NEAR, "near")):
+        out = t.fmm(meta["p"], q, O.POTGRAD, which, self_lists=(row, val))
+        assert rel_l2(out[idx, 0], g[f"phi_{suffix}"]) < 1e-12, suffix
+        assert rel_l2(out[idx, 1:], g[f"grad_{suffix}"]) < 1e-12, suffix
+    # the lists matter: without them the near field differs
+    plain = t.fmm(meta["p"], q, O.POTGRAD, O.NEAR)
+    assert rel_l2(plain[idx, 0], g["phi_near"]) > 1e-6
+
+
+def test_reference_error_recorded_against_direct_sum():
+    # the fixtures that carry direct sums also carry the reference's own error
+    # against them; it reproduces from the stored arrays
+    for case in golden_cases(None):
+        meta, g, _, _ = load_golden(case)
+        if "phi_direct" not in g:
+            continue
+        err = meta["reference_error_vs_direct"]
+        assert abs(rel_l2(g["phi_full"], g["phi_direct"]) - err["phi"]) < 1e-3 * err["phi"]
+        if "grad_direct" in g:
+            assert abs(rel_l2(g["grad_full"], g["grad_direct"]) - err["grad"]) < 1e-3 * err["grad"]
 
 
 def test_curl_identity_in_fixtures():

@@ -141,3 +141,16 @@ def test_tree_errors():
     bad[3, 1] = np.nan
     with pytest.raises(api.SfmmError):
         api.OctreePartitioner(bad, api.Box(corner=(0, 0, 0), width=1.0)).refine(20, 8)
+
+
+def test_near_sources_are_the_near_list():
+    # sfmm_tree_near_sources: the particles N2N visits for target i (fmm.hpp:779-785)
+    pos, _ = W.plummer(3000, seed=82)
+    part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, 16)
+    t = O.Tree(pos, 20, 16)
+    for i in range(0, 3000, 211):
+        src = part.near_sources(i)
+        assert i in src and len(set(src)) == len(src)
+        assert sorted(src) == [j for j in range(3000) if t.is_neighbor(i, j)]
+    with pytest.raises(api.SfmmError):
+        part.near_sources(3000)
