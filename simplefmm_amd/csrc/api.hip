@@ -7,6 +7,7 @@
 // L2P; rinv adds the near field (N2N, fmm.hpp:773-809).
 #include "../../include/sfmm.h"
 #include "kernels.hpp"
+#include "m2l_dense.hpp"
 #include "tables.hpp"
 #include "tree.hpp"
 
@@ -156,6 +157,16 @@ struct sfmm_plan_s {
     const double* m2m_frag = nullptr;   // matrix-core M2M / L2L operators [8][nfrag][64]
     const double* l2l_frag = nullptr;
     sfmm::TransArgs m2m[5], l2l[5];     // per component count (built on first use)
+    // mixed precision, p <= 12: M2L on the f16 matrix cores (m2l_dense.hpp)
+    bool dense = false;
+    sfmm::DenseArgs dns[5];             // per component count (built on first use)
+    const void* dense_ops = nullptr;    // class operators, f16 hi/lo fragments
+    const float* dense_az = nullptr;    // cos, sin of m phi per azimuth group
+    const float* dense_zero = nullptr;  // zero columns of absent sources
+    float* dense_nodemax = nullptr;     // largest |value| per node and component of the FP32 multipoles
+    int dense_op_exp = 0;
+    int dense_kd = 0;                   // floats per (node, component) of the FP32 copy
+    std::vector<int32_t> dense_grp;     // azimuth group per offset id
     sfmm::SelfCorr self;                // self-list corrections (empty without a self generator)
     std::vector<int64_t> slist_row, slist_val;  // far self pairs per particle, original order (_slist)
     std::vector<int64_t> m2m_lvl[5], l2l_lvl[5];  // first block of each level (+ end)
@@ -256,8 +267,14 @@ int ensure_buffers(sfmm_plan_s* P, int nc, int qdim)
         const size_t bytes = sizeof(double) * static_cast<size_t>(nnode) * nc * P->K;
         if (dev_malloc(&P->M, bytes) != hipSuccess || dev_malloc(&P->L, bytes) != hipSuccess)
             return fail(SFMM_ERR_ALLOC, "hipMalloc failed (expansions)");
-        if (P->precision == SFMM_MIXED && P->m2l_kernel == 0 && dev_malloc(&P->M32, bytes / 2) != hipSuccess)
-            return fail(SFMM_ERR_ALLOC, "hipMalloc failed (FP32 multipoles)");
+        if (P->precision == SFMM_MIXED && P->m2l_kernel == 0) {
+            // the dense layout pads each (node, component) to dense_kd floats; the
+            // padding stays zero (P2M and M2M write only the coefficients)
+            const int row = P->dense ? P->dense_kd : P->K;
+            const size_t b32 = sizeof(float) * static_cast<size_t>(nnode) * nc * row;
+            if (dev_malloc(&P->M32, b32) != hipSuccess) return fail(SFMM_ERR_ALLOC, "hipMalloc failed (FP32 multipoles)");
+            HIP_TRY(hipMemset(P->M32, 0, b32));
+        }
         P->nc_alloc = nc;
     }
     if (qdim > P->qdim_alloc) {
@@ -281,31 +298,17 @@ int ensure_buffers(sfmm_plan_s* P, int nc, int qdim)
     return SFMM_OK;
 }
 
-// Groups the targets of each level by child octant into blocks of 16/nc and
-// lists, per block, the offsets that occur with the source of every target.
-int build_m2l_blocks(sfmm_plan_s* P, int nc)
+// Targets of each level grouped by child octant into blocks of TB (per level,
+// the q-th chunks of the 8 octants, children of nearby parents, are adjacent);
+// per block, the offsets that occur with any of its targets, in `order`, each
+// with meta(offset) and the source of every target (-1: none).  Blocks are cut
+// on all host threads; fails with more than 189 offsets in a block.
+int cut_m2l_blocks(sfmm_plan_s* P, int TB, const std::vector<int>& order, const std::function<int32_t(int)>& meta,
+                   std::vector<int32_t>& blk_tgt, std::vector<int64_t>& blk_row, std::vector<int32_t>& ent_meta,
+                   std::vector<int32_t>& ent_src)
 {
-    // built into a local and published only when every upload succeeded, so a
-    // failed attempt leaves the plan without blocks and the next one retries
-    if (P->m2l[nc].blk_row) return SFMM_OK;
-    sfmm::M2LArgs A;
     const sfmm::Tree& t = P->tree->t;
-    const int TB = 16 / nc;
-    std::vector<int32_t> blk_tgt, ent_src;
-    std::vector<int64_t> blk_row{0};
-    std::vector<int32_t> ent_meta;
-    // offsets in (polar class, distance class, id) order: consecutive offsets
-    // share the rotation operators, and mostly the coaxial one as well
-    const auto& cls = P->tables.off_class;
-    const auto& d2 = P->tables.off_d2;
-    std::vector<int> order;
-    for (int off = 0; off < 343; ++off)
-        if (cls[off] >= 0) order.push_back(off);
-    std::sort(order.begin(), order.end(), [&](int x, int y) {
-        return cls[x] != cls[y] ? cls[x] < cls[y] : d2[x] != d2[y] ? d2[x] < d2[y] : x < y;
-    });
-    // blocks in launch order: per level, the q-th chunks of the 8 octants
-    // (children of nearby parents) are adjacent
+    blk_row.assign(1, 0);
     struct Blk {
         int level, octant;
         int64_t first;  // index of the first target in the level's octant list
@@ -321,13 +324,12 @@ int build_m2l_blocks(sfmm_plan_s* P, int nc)
             for (int o = 0; o < 8; ++o)
                 if (q * TB < oct_nodes[8 * l + o].size()) blocks.push_back({l, o, static_cast<int64_t>(q * TB)});
     }
-    // each thread fills a contiguous range of blocks; the ranges are concatenated in order
     const int nth = host_threads();
     std::vector<std::vector<int32_t>> t_tgt(nth), t_meta(nth), t_src(nth);
     std::vector<std::vector<int64_t>> t_cnt(nth);
     std::vector<char> t_bad(nth, 0);
     parallel_ranges(static_cast<int64_t>(blocks.size()), nth, [&](int64_t b0, int64_t b1, int c) {
-        std::vector<int32_t> src(343 * TB);
+        std::vector<int32_t> src(static_cast<size_t>(343) * TB);
         std::vector<char> used(343, 0);
         for (int64_t bi = b0; bi < b1; ++bi) {
             const Blk& B = blocks[bi];
@@ -343,7 +345,7 @@ int build_m2l_blocks(sfmm_plan_s* P, int nc)
                 t_tgt[c].push_back(static_cast<int32_t>(P->level_first[B.level] + a));
                 for (int64_t k = lv.il_row[a]; k < lv.il_row[a + 1]; ++k) {
                     const int off = lv.il_off[k];
-                    src[off * TB + tt] = static_cast<int32_t>(P->level_first[B.level] + lv.il_src[k]);
+                    src[static_cast<size_t>(off) * TB + tt] = static_cast<int32_t>(P->level_first[B.level] + lv.il_src[k]);
                     used[off] = 1;
                 }
             }
@@ -351,8 +353,9 @@ int build_m2l_blocks(sfmm_plan_s* P, int nc)
             for (int off : order) {
                 if (!used[off]) continue;
                 used[off] = 0;
-                t_meta[c].push_back(off | cls[off] << 9 | d2[off] << 16);
-                t_src[c].insert(t_src[c].end(), src.begin() + off * TB, src.begin() + (off + 1) * TB);
+                t_meta[c].push_back(meta(off));
+                t_src[c].insert(t_src[c].end(), src.begin() + static_cast<size_t>(off) * TB,
+                                src.begin() + static_cast<size_t>(off + 1) * TB);
                 ++cnt;
             }
             if (cnt > 189) t_bad[c] = 1;
@@ -367,6 +370,69 @@ int build_m2l_blocks(sfmm_plan_s* P, int nc)
         ent_src.insert(ent_src.end(), t_src[c].begin(), t_src[c].end());
         for (int64_t v : t_cnt[c]) blk_row.push_back(blk_row.back() + v);
     }
+    return SFMM_OK;
+}
+
+// Blocks of the mixed-precision M2L on the f16 matrix cores: 32 targets per
+// wave group (4 / waves-per-group groups per block), offsets in (azimuth
+// group, polar class, id) order so that each azimuth group is one run.
+int build_dense_blocks(sfmm_plan_s* P, int nc)
+{
+    if (P->dns[nc].blk_row) return SFMM_OK;
+    const int ncw = nc == 3 ? 4 : nc;
+    const int TB = 32 * (4 / ncw);
+    const auto& cls = P->tables.off_class;
+    const auto& grp = P->dense_grp;
+    std::vector<int> order;
+    for (int off = 0; off < 343; ++off)
+        if (cls[off] >= 0) order.push_back(off);
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        return grp[x] != grp[y] ? grp[x] < grp[y] : cls[x] != cls[y] ? cls[x] < cls[y] : x < y;
+    });
+    std::vector<int32_t> blk_tgt, ent_meta, ent_src;
+    std::vector<int64_t> blk_row;
+    if (int rc = cut_m2l_blocks(P, TB, order, [&](int off) { return cls[off] | grp[off] << 8; }, blk_tgt, blk_row,
+                                ent_meta, ent_src))
+        return rc;
+    sfmm::DenseArgs A;
+    A.n_block = static_cast<int64_t>(blk_row.size()) - 1;
+    int rc;
+    if ((rc = P->upload(blk_tgt, &A.blk_tgt)) || (rc = P->upload(blk_row, &A.blk_row)) ||
+        (rc = P->upload(ent_meta, &A.ent_meta)) || (rc = P->upload(ent_src, &A.ent_src)))
+        return rc;
+    A.ops = P->dense_ops;
+    A.az = P->dense_az;
+    A.nodemax = P->dense_nodemax;
+    A.zero = P->dense_zero;
+    A.op_exp = P->dense_op_exp;
+    P->dns[nc] = A;
+    return SFMM_OK;
+}
+
+// Groups the targets of each level by child octant into blocks of 16/nc and
+// lists, per block, the offsets that occur with the source of every target.
+int build_m2l_blocks(sfmm_plan_s* P, int nc)
+{
+    // built into a local and published only when every upload succeeded, so a
+    // failed attempt leaves the plan without blocks and the next one retries
+    if (P->m2l[nc].blk_row) return SFMM_OK;
+    sfmm::M2LArgs A;
+    const int TB = 16 / nc;
+    // offsets in (polar class, distance class, id) order: consecutive offsets
+    // share the rotation operators, and mostly the coaxial one as well
+    const auto& cls = P->tables.off_class;
+    const auto& d2 = P->tables.off_d2;
+    std::vector<int> order;
+    for (int off = 0; off < 343; ++off)
+        if (cls[off] >= 0) order.push_back(off);
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        return cls[x] != cls[y] ? cls[x] < cls[y] : d2[x] != d2[y] ? d2[x] < d2[y] : x < y;
+    });
+    std::vector<int32_t> blk_tgt, ent_meta, ent_src;
+    std::vector<int64_t> blk_row;
+    if (int rc = cut_m2l_blocks(P, TB, order, [&](int off) { return off | cls[off] << 9 | d2[off] << 16; }, blk_tgt,
+                                blk_row, ent_meta, ent_src))
+        return rc;
     A.n_block = static_cast<int64_t>(blk_row.size()) - 1;
     int rc;
     if ((rc = P->upload(blk_tgt, &A.blk_tgt)) || (rc = P->upload(blk_row, &A.blk_row)) ||
@@ -482,7 +548,7 @@ int prepare(sfmm_plan_s* P, int kind, int ncharge, int which, sfmm::Mode& mode, 
     HIP_TRY(hipSetDevice(P->device));
     if (int rc = ensure_buffers(P, nc, qdim)) return rc;
     if (P->m2l_kernel == 0) {
-        if (int rc = build_m2l_blocks(P, nc)) return rc;
+        if (int rc = P->dense ? build_dense_blocks(P, nc) : build_m2l_blocks(P, nc)) return rc;
         if (int rc = build_translate_blocks(P, nc)) return rc;
     }
     return SFMM_OK;
@@ -516,7 +582,7 @@ int run_on(sfmm_plan_s* P, sfmm::Mode mode, int nc, int qdim, const double* dQ, 
     REC(1);
     const bool far = (which & SFMM_FAR) && depth >= 2;  // fmm.hpp:816-820
     // the mixed M2L reads an FP32 copy of the multipoles, written by P2M and M2M
-    float* M32 = P->m2l_kernel == 0 && P->m2l[nc].fp32 ? P->M32 : nullptr;
+    float* M32 = P->m2l_kernel == 0 && (P->dense || P->m2l[nc].fp32) ? P->M32 : nullptr;
     if (far) sfmm::launch_p2m(d, mode, nc, P->qbuf, P->M, M32, s);
     REC(2);
     sfmm::TransArgs m2m = P->m2m[nc];
@@ -530,7 +596,13 @@ int run_on(sfmm_plan_s* P, sfmm::Mode mode, int nc, int qdim, const double* dQ, 
         }
     REC(3);
     if (far) {
-        if (P->m2l_kernel == 0) {
+        if (P->dense) {
+            // range of every FP32 multipole (levels >= 2), which scales the columns, then M2L
+            const int64_t f2 = P->level_first[2];
+            sfmm::launch_dense_nodemax(M32 + f2 * nc * P->dense_kd, P->level_first.back() - f2, nc, P->dense_kd,
+                                       P->dense_nodemax + f2 * nc, s);
+            sfmm::launch_m2l_dense(P->dns[nc], P->p, nc, M32, P->L, s);
+        } else if (P->m2l_kernel == 0) {
             sfmm::M2LArgs& A = P->m2l[nc];
             if (A.fp32) A.M32 = P->M32;
             sfmm::launch_m2l_mfma(A, P->p, nc, P->M, P->L, s);
@@ -1041,7 +1113,22 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
         (rc = P->upload(sp_slot, &P->p2p.split_slot)) || (rc = P->upload(sp_first, &P->p2p.split_first)) ||
         (rc = P->upload(sp_nitem, &P->p2p.split_nitem)) || (rc = P->upload(sp_stride, &P->p2p.split_stride)))
         return rc;
-    {
+    P->dense = P->precision == SFMM_MIXED && P->m2l_kernel == 0 && p <= sfmm::kDenseMaxP;
+    if (P->dense) {
+        std::vector<uint16_t> ops;
+        std::vector<float> az;
+        sfmm::build_dense_m2l(tb, ops, P->dense_op_exp, az, P->dense_grp);
+        P->dense_kd = sfmm::dense_order(p).kd;
+        const uint16_t* dops = nullptr;
+        const float* dmax = nullptr;
+        if ((rc = P->upload(ops, &dops)) || (rc = P->upload(az, &P->dense_az)) ||
+            (rc = P->upload(std::vector<float>(P->dense_kd, 0.0f), &P->dense_zero)) ||
+            (rc = P->upload(std::vector<float>(static_cast<size_t>(4) * nnode, 0.0f), &dmax)))
+            return rc;
+        P->dense_ops = dops;
+        P->dense_nodemax = const_cast<float*>(dmax);
+    }
+    if (!P->dense) {
         std::vector<double> rotA, coaxA;
         sfmm::build_m2l_fragments(tb, P->precision == SFMM_MIXED, rotA, coaxA);
         if (P->precision == SFMM_MIXED) {
@@ -1050,8 +1137,8 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
         } else {
             if ((rc = P->upload(rotA, &P->rotA64)) || (rc = P->upload(coaxA, &P->coaxA64))) return rc;
         }
-        if ((rc = P->upload(std::vector<double>(4 * P->K, 0.0), &P->zero_cols))) return rc;
     }
+    if ((rc = P->upload(std::vector<double>(4 * P->K, 0.0), &P->zero_cols))) return rc;
     {
         // matrix-core M2M / L2L operators from the row-major per-octant tables
         const int K = P->K;

@@ -9,6 +9,7 @@
 // b = sqrt((n-1)^2-m^2)/sqrt(n^2-m^2).  R(n,m) equals the reference's
 // rho^n Z(n,m,theta) e^{-i m phi} (fmm.hpp:107-111,226-232,537-548).
 #include "kernels.hpp"
+#include "m2l_dense.hpp"
 
 #include <type_traits>
 
@@ -155,10 +156,11 @@ __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int nc, co
     __syncthreads();
 
     double* Mg = M + static_cast<int64_t>(gnode) * nc * K;
-    float* Mg32 = M32 ? M32 + static_cast<int64_t>(gnode) * nc * K : nullptr;  // FP32 copy for the mixed M2L
-    auto put = [&](int i, double v) {
-        Mg[i] = v;
-        if (Mg32) Mg32[i] = static_cast<float>(v);
+    using FL = F32LayoutOf<P>;  // FP32 copy for the mixed M2L
+    float* Mg32 = M32 ? M32 + static_cast<int64_t>(gnode) * nc * FL::L.row : nullptr;
+    auto put = [&](int c, int r, double v) {
+        Mg[c * K + r] = v;
+        if (Mg32) Mg32[c * FL::L.row + FL::L.pos[r]] = static_cast<float>(v);
     };
     for (int t = tid; t < NT; t += 64) {
         // decode (n, mm) of triangle index t
@@ -167,8 +169,8 @@ __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int nc, co
         const int mm = t - n * (n + 1) / 2;
         if (mode == MODE_POT) {
             for (int c = 0; c < nc; ++c) {
-                put(c * K + rfc(n, mm), S[c][2 * t]);
-                if (mm > 0) put(c * K + rfs(n, mm), S[c][2 * t + 1]);
+                put(c, rfc(n, mm), S[c][2 * t]);
+                if (mm > 0) put(c, rfs(n, mm), S[c][2 * t + 1]);
             }
             continue;
         }
@@ -225,8 +227,8 @@ __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int nc, co
             nout = 3;
         }
         for (int c = 0; c < nout; ++c) {
-            put(c * K + rfc(n, mm), out[c][0]);
-            if (mm > 0) put(c * K + rfs(n, mm), out[c][1]);
+            put(c, rfc(n, mm), out[c][0]);
+            if (mm > 0) put(c, rfs(n, mm), out[c][1]);
         }
     }
 }
@@ -496,10 +498,9 @@ __device__ __forceinline__ void pair_kernel(double dx, double dy, double dz, con
     }
 }
 
-// Mixed-precision near field: coordinates are taken relative to the target
-// leaf's first particle in FP64 and rounded to FP32 once per staged source, the
-// pair arithmetic is FP32 (v_rsq_f32), and each target's sum over a source tile
-// of 64 is added to an FP64 accumulator.
+// Mixed-precision near field: the pair's coordinate differences arrive in
+// FP32 (taken in FP64), the pair arithmetic is FP32 (v_rsq_f32), and each
+// target's sum over a source tile of 64 is added to an FP64 accumulator.
 template <int MODE, int NCQ>
 __device__ __forceinline__ void pair_kernel_f32(float dx, float dy, float dz, const float* qj, float* acc)
 {
@@ -529,18 +530,22 @@ __device__ __forceinline__ void pair_kernel_f32(float dx, float dy, float dz, co
 // long (coarse leaves next to deep subtrees) are split over several items so
 // that no wavefront carries a long tail.  Lanes are (target, split): a leaf of
 // nt < 64 targets spreads each target over floor(64/nt) lanes.  Sources are
-// staged through LDS 64 at a time.  An unsplit item adds into the output; the
-// items of a split leaf write partial sums that p2p_reduce_kernel adds in a
-// fixed order, so results are bitwise repeatable.
-// MIXED: coordinates relative to the target leaf's first particle, rounded to
-// FP32 once per staged source; FP32 pair arithmetic; FP64 accumulation per tile.
+// staged through LDS 64 at a time; only tiles that hold particles of the
+// target leaf itself test for the pair (i, i) (the reference's default
+// is_self, fmm.hpp:875-881).  An unsplit item adds into the output; the items
+// of a split leaf write partial sums that p2p_reduce_kernel adds in a fixed
+// order, so results are bitwise repeatable.
+// MIXED: coordinate differences in FP64 rounded to FP32 (so close pairs keep
+// their relative precision), FP32 pair arithmetic, FP32 sums per source tile
+// added into FP64 accumulators.
 template <int MODE, int NCQ, int NOUT, bool MIXED>
 __global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, P2PItems w, const double* __restrict__ q,
                                                  double* __restrict__ out, double* __restrict__ partial)
 {
     using R = typename std::conditional<MIXED, float, double>::type;
-    __shared__ R sx[64], sy[64], sz[64], sq[NCQ][64];
-    __shared__ int64_t sidx[64];
+    __shared__ double sx[64], sy[64], sz[64];
+    __shared__ R sq[NCQ][64];
+    __shared__ int32_t sidx[64];
     __shared__ double red[NOUT][64];
     const int64_t it = blockIdx.x;
     const int tid = threadIdx.x;
@@ -550,7 +555,6 @@ __global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, P2PItems w, const do
     const int64_t tb = d.lf_begin[g], te = d.lf_end[g];
     const int64_t nt = te - tb;
     const int64_t r0 = d.near_row[g], r1 = d.near_row[g + 1];
-    const double ox = MIXED ? d.px[tb] : 0.0, oy = MIXED ? d.py[tb] : 0.0, oz = MIXED ? d.pz[tb] : 0.0;
     const int ntl = nt < 64 ? static_cast<int>(nt) : 64;  // targets per pass
     const int S = 64 / ntl;
     const int t = tid % ntl, s = tid / ntl;
@@ -558,12 +562,13 @@ __global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, P2PItems w, const do
     for (int64_t t0 = tb; t0 < te; t0 += ntl) {
         const int64_t slot = t0 + t;
         const bool valid = lane_on && slot < te;
-        R xi = 0, yi = 0, zi = 0;
+        double xi = 0, yi = 0, zi = 0;
         if (valid) {
-            xi = static_cast<R>(d.px[slot] - ox);
-            yi = static_cast<R>(d.py[slot] - oy);
-            zi = static_cast<R>(d.pz[slot] - oz);
+            xi = d.px[slot];
+            yi = d.py[slot];
+            zi = d.pz[slot];
         }
+        const int32_t slot32 = static_cast<int32_t>(slot);
         double acc[NOUT];
 #pragma unroll
         for (int c = 0; c < NOUT; ++c) acc[c] = 0.0;
@@ -571,30 +576,42 @@ __global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, P2PItems w, const do
         for (int64_t ut = u0; ut < u1; ut += 64) {
             __syncthreads();
             const int64_t u = ut + tid;
+            bool mine = false;
             if (u < u1) {
                 while (rc + 1 < r1 && d.near_pref[rc + 1] <= u) ++rc;
                 const int64_t j = d.near_begin[rc] + (u - d.near_pref[rc]);
-                sidx[tid] = j;
-                sx[tid] = static_cast<R>(d.px[j] - ox);
-                sy[tid] = static_cast<R>(d.py[j] - oy);
-                sz[tid] = static_cast<R>(d.pz[j] - oz);
+                mine = j >= tb && j < te;
+                sidx[tid] = static_cast<int32_t>(j);
+                sx[tid] = d.px[j];
+                sy[tid] = d.py[j];
+                sz[tid] = d.pz[j];
 #pragma unroll
                 for (int c = 0; c < NCQ; ++c) sq[c][tid] = static_cast<R>(q[c * d.n + j]);
             }
+            const bool own = __any(mine);  // the tile holds particles of the target leaf
             __syncthreads();
             const int cnt = static_cast<int>(u1 - ut < 64 ? u1 - ut : 64);
             if (valid) {
                 R a[NOUT];
 #pragma unroll
                 for (int c = 0; c < NOUT; ++c) a[c] = 0;
-                for (int k = s; k < cnt; k += S) {
-                    if (sidx[k] == slot) continue;
-                    R qj[NCQ];
+                auto pairs = [&](auto checked) {
+                    for (int k = s; k < cnt; k += S) {
+                        if constexpr (decltype(checked)::value)
+                            if (sidx[k] == slot32) continue;
+                        R qj[NCQ];
 #pragma unroll
-                    for (int c = 0; c < NCQ; ++c) qj[c] = sq[c][k];
-                    if constexpr (MIXED) pair_kernel_f32<MODE, NCQ>(xi - sx[k], yi - sy[k], zi - sz[k], qj, a);
-                    else pair_kernel<MODE, NCQ>(xi - sx[k], yi - sy[k], zi - sz[k], qj, a);
-                }
+                        for (int c = 0; c < NCQ; ++c) qj[c] = sq[c][k];
+                        const double dx = xi - sx[k], dy = yi - sy[k], dz = zi - sz[k];
+                        if constexpr (MIXED)
+                            pair_kernel_f32<MODE, NCQ>(static_cast<float>(dx), static_cast<float>(dy),
+                                                       static_cast<float>(dz), qj, a);
+                        else
+                            pair_kernel<MODE, NCQ>(dx, dy, dz, qj, a);
+                    }
+                };
+                if (own) pairs(std::true_type{});
+                else pairs(std::false_type{});
 #pragma unroll
                 for (int c = 0; c < NOUT; ++c) acc[c] += static_cast<double>(a[c]);
             }

@@ -71,6 +71,22 @@ struct M2LArgs {
     int fp32 = 0;                      // 1: FP32 MFMA stages (mixed precision)
 };
 
+// Mixed-precision M2L on the f16 matrix cores (m2l_dense.hpp): blocks of
+// targets of one level and child octant; per block the interaction offsets of
+// its targets grouped by azimuth, each with one source per target.
+struct DenseArgs {
+    int64_t n_block = 0;
+    const int32_t* blk_tgt = nullptr;   // [n_block][targets per block] target gnodes, -1 = none
+    const int64_t* blk_row = nullptr;   // [n_block+1] entry ranges
+    const int32_t* ent_meta = nullptr;  // per entry: polar class | azimuth group << 8
+    const int32_t* ent_src = nullptr;   // [entries][targets per block] source gnodes, -1 = none
+    const void* ops = nullptr;          // per class: [fragment][hi 64 | lo 64] x 8 halves (f16)
+    const float* az = nullptr;          // [azimuth group][p+1][2] cos, sin of m phi
+    const float* nodemax = nullptr;     // [gnode][nc] largest |value| of each FP32 multipole component
+    const float* zero = nullptr;        // zeros: the columns of absent sources
+    int op_exp = 0;                     // the operators are stored scaled by 2^op_exp
+};
+
 // Near-field work: items of (leaf, run of its concatenated near sources).
 struct P2PItems {
     int64_t n_item = 0;
@@ -124,6 +140,10 @@ void launch_m2l(const DevPlan& d, int nc, const double* M, double* L, hipStream_
 void launch_l2l(const DevPlan& d, LevelSpan children, int nc, double* L, hipStream_t s);
 void launch_l2p(const DevPlan& d, int nc, const double* L, double* out, hipStream_t s);
 void launch_m2l_mfma(const M2LArgs& a, int p, int nc, const double* M, double* L, hipStream_t s);
+// p <= 12; M32: the multipoles in FP32, [gnode][nc][dense_order(p).kd] (m2l_dense.hpp)
+void launch_m2l_dense(const DenseArgs& a, int p, int nc, const float* M32, double* L, hipStream_t s);
+// nodemax[node * nc + c] = largest |M32| of component c of each of `count` nodes of nc x kd floats
+void launch_dense_nodemax(const float* M32, int64_t count, int nc, int kd, float* nodemax, hipStream_t s);
 // upper = false: M2M (parents written); true: L2L (children added to).
 void launch_translate(const TransArgs& a, LevelSpan blocks, int p, int nc, bool upper, double* io, hipStream_t s);
 int translate_nfrag(int p, bool upper);

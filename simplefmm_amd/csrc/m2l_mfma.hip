@@ -7,7 +7,8 @@ void launch_m2l_mfma(const M2LArgs& a, int p, int nc, const double* M, double* L
 {
     if (a.n_block == 0) return;
     if (a.fp32) {
-        if (!m2l_f32_lo(a, p, nc, M, L, s) && !m2l_f32_hi(a, p, nc, M, L, s)) m2l_f32_top(a, p, nc, M, L, s);
+        // orders 13..16; the mixed path of lower orders runs m2l_dense.hpp
+        (void)m2l_f32_top(a, p, nc, M, L, s);
     } else {
         if (!m2l_f64_lo(a, p, nc, M, L, s) && !m2l_f64_hi(a, p, nc, M, L, s)) m2l_f64_top(a, p, nc, M, L, s);
     }

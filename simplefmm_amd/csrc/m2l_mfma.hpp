@@ -1001,8 +1001,6 @@ bool m2l_dispatch(const M2LArgs& a, int p, int nc, const double* M, double* L, h
 bool m2l_f64_lo(const M2LArgs& a, int p, int nc, const double* M, double* L, hipStream_t s);
 bool m2l_f64_hi(const M2LArgs& a, int p, int nc, const double* M, double* L, hipStream_t s);
 bool m2l_f64_top(const M2LArgs& a, int p, int nc, const double* M, double* L, hipStream_t s);
-bool m2l_f32_lo(const M2LArgs& a, int p, int nc, const double* M, double* L, hipStream_t s);
-bool m2l_f32_hi(const M2LArgs& a, int p, int nc, const double* M, double* L, hipStream_t s);
 bool m2l_f32_top(const M2LArgs& a, int p, int nc, const double* M, double* L, hipStream_t s);
 
 }  // namespace sfmm

@@ -6,8 +6,10 @@
 // restated here in the level-scaled real form.
 #include "tables.hpp"
 
+#include "m2l_dense.hpp"
 #include "m2l_sched.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 
@@ -338,6 +340,136 @@ void build_tables(int p, Tables& t)
             t.om[rf_cos(n, k)] = ooff[k] + (n - k);
             t.om[rf_sin(n, k)] = ooff[k] + (p + 1 - k) + (n - k);
         }
+    }
+}
+
+uint16_t f16_bits(double x)
+{
+    const uint16_t sign = std::signbit(x) ? 0x8000 : 0;
+    const double a = std::fabs(x);
+    if (a == 0.0) return sign;
+    int e;
+    (void)std::frexp(a, &e);  // a in [2^(e-1), 2^e)
+    int E = std::max(e - 1, -14);  // below 2^-14: subnormal, quantum 2^-24
+    int64_t m = static_cast<int64_t>(std::nearbyint(std::ldexp(a, 10 - E)));
+    if (m == 2048) {
+        ++E;
+        m = 1024;
+    }
+    if (m < 1024) return static_cast<uint16_t>(sign | m);
+    return static_cast<uint16_t>(sign | ((E + 15) << 10) | (m - 1024));
+}
+
+double f16_value(uint16_t b)
+{
+    const int E = (b >> 10) & 31, m = b & 1023;
+    const double v = E == 0 ? std::ldexp(static_cast<double>(m), -24) : std::ldexp(1024.0 + m, E - 25);
+    return (b & 0x8000) ? -v : v;
+}
+
+void build_dense_m2l(const Tables& t, std::vector<uint16_t>& ops, int& op_exp, std::vector<float>& az,
+                     std::vector<int32_t>& off_grp)
+{
+    const int P = t.p;
+    const DenseOrder O = dense_order(P);
+    // the polar classes of interaction offsets (not adjacent: some |d_i| >= 2) and
+    // their |d|^2; a class holds only adjacent or only non-adjacent offsets, and
+    // the adjacent ones (|d| < 2, never translated) would set the scale
+    std::vector<int> cls_d2(70, -1);
+    for (int id = 0; id < 343; ++id) {
+        const int di = id % 7 - 3, dj = (id / 7) % 7 - 3, dk = id / 49 - 3;
+        if (t.off_class[id] >= 0 && std::max({std::abs(di), std::abs(dj), std::abs(dk)}) >= 2)
+            cls_d2[t.off_class[id]] = t.off_d2[id];
+    }
+    // dense blocks per class: part 0 cos [kc][kc], part 1 sin [ks][ks]
+    std::vector<double> D(static_cast<size_t>(70) * (O.kc * O.kc + O.ks * O.ks), 0.0);
+    double dmax = 0.0;
+    for (int c = 0; c < 70; ++c) {
+        if (cls_d2[c] < 1) continue;
+        const double* fwd = t.rot.data() + static_cast<size_t>(c) * 2 * t.rot_size;
+        const double* bwd = fwd + t.rot_size;
+        const double* C = t.coax.data() + static_cast<size_t>(cls_d2[c]) * t.coax_size;
+        double* Dc = D.data() + static_cast<size_t>(c) * (O.kc * O.kc + O.ks * O.ks);
+        for (int part = 0; part < 2; ++part) {
+            const int ext = part == 0 ? O.kc : O.ks;
+            double* Dp = Dc + (part == 0 ? 0 : O.kc * O.kc);
+            for (int kr = 0; kr < ext; ++kr) {
+                const int er = dense_elem(O, part, kr);
+                if (er < 0) continue;
+                const int j = er / 32, a = er % 32;
+                for (int kk = 0; kk < ext; ++kk) {
+                    const int ec = dense_elem(O, part, kk);
+                    if (ec < 0) continue;
+                    const int n = ec / 32, b = ec % 32;
+                    double v = 0.0;
+                    for (int q = part; q <= std::min(j, n); ++q) {
+                        const int w = P + 1 - q;
+                        const double tq = C[t.coax_off[q] + (j - q) * w + (n - q)];
+                        if (part == 0)
+                            v += bwd[t.rot_off[j] + a * (j + 1) + q] * tq * fwd[t.rot_off[n] + q * (n + 1) + b];
+                        else
+                            v += bwd[t.rot_off[j] + (j + 1) * (j + 1) + (a - 1) * j + (q - 1)] * tq *
+                                 fwd[t.rot_off[n] + (n + 1) * (n + 1) + (q - 1) * n + (b - 1)];
+                    }
+                    Dp[kr * ext + kk] = v;
+                    dmax = std::max(dmax, std::fabs(v));
+                }
+            }
+        }
+    }
+    int e = 0;
+    if (dmax > 0.0) (void)std::frexp(dmax, &e);  // dmax < 2^e
+    op_exp = 14 - e;
+    // fragments
+    const size_t per_class = static_cast<size_t>(O.nfrag) * 1024;
+    ops.assign(per_class * 70, 0);
+    for (int c = 0; c < 70; ++c) {
+        const double* Dc = D.data() + static_cast<size_t>(c) * (O.kc * O.kc + O.ks * O.ks);
+        for (int part = 0; part < 2; ++part) {
+            const int ext = part == 0 ? O.kc : O.ks, mts = part == 0 ? O.mtc : O.mts, kss = part == 0 ? O.ksc : O.kss;
+            const double* Dp = Dc + (part == 0 ? 0 : O.kc * O.kc);
+            for (int mt = 0; mt < mts; ++mt)
+                for (int ks = 0; ks < kss; ++ks) {
+                    const int f = part == 0 ? mt * O.ksc + ks : O.mtc * O.ksc + mt * O.kss + ks;
+                    uint16_t* hi = ops.data() + per_class * c + static_cast<size_t>(f) * 1024;
+                    uint16_t* lo = hi + 512;
+                    for (int l = 0; l < 64; ++l)
+                        for (int jj = 0; jj < 8; ++jj) {
+                            const int row = 32 * mt + l % 32, col = 16 * ks + 8 * (l / 32) + jj;
+                            const double x = (row < ext && col < ext) ? std::ldexp(Dp[row * ext + col], op_exp) : 0.0;
+                            const uint16_t bh = f16_bits(x);
+                            hi[l * 8 + jj] = bh;
+                            lo[l * 8 + jj] = f16_bits(x - f16_value(bh));
+                        }
+                }
+        }
+    }
+    // azimuth groups: the primitive directions (a, b) of (di, dj), (0, 0) with (1, 0)
+    std::vector<std::pair<int, int>> dirs;
+    off_grp.assign(343, -1);
+    az.assign(static_cast<size_t>(kAzGroups) * (P + 1) * 2, 0.0f);
+    for (int id = 0; id < 343; ++id) {
+        if (t.off_class[id] < 0) continue;
+        int di = id % 7 - 3, dj = (id / 7) % 7 - 3;
+        if (di == 0 && dj == 0) di = 1;
+        int g0 = std::abs(di), g1 = std::abs(dj);
+        while (g1) {
+            const int r = g0 % g1;
+            g0 = g1;
+            g1 = r;
+        }
+        const std::pair<int, int> dir{di / g0, dj / g0};
+        int g = static_cast<int>(std::find(dirs.begin(), dirs.end(), dir) - dirs.begin());
+        if (g == static_cast<int>(dirs.size())) {
+            dirs.push_back(dir);
+            for (int m = 0; m <= P; ++m) {
+                az[(static_cast<size_t>(g) * (P + 1) + m) * 2] =
+                    static_cast<float>(t.off_cs[(static_cast<size_t>(id) * (P + 1) + m) * 2]);
+                az[(static_cast<size_t>(g) * (P + 1) + m) * 2 + 1] =
+                    static_cast<float>(t.off_cs[(static_cast<size_t>(id) * (P + 1) + m) * 2 + 1]);
+            }
+        }
+        off_grp[id] = g;
     }
 }
 

@@ -54,6 +54,22 @@ struct Tables {
 void build_m2l_fragments(const Tables& t, bool interleave_rows, std::vector<double>& rotA,
                          std::vector<double>& coaxA);
 
+// Operators of the mixed-precision M2L on the f16 matrix cores (m2l_dense.hpp),
+// p <= 12.  Per polar class c, the azimuth-aligned operator D_c = R_c^T T R_c
+// (back rotation, coaxial translation, forward rotation) in its cos and sin
+// blocks, rows and columns in dense_order(p), scaled by 2^op_exp (largest entry
+// below 2^14) and split as hi = f16(x), lo = f16(x - hi); stored as the A
+// fragments of 32x32x16 MFMA tiles, ops[class][fragment][hi, lo][64 lanes][8]
+// with lane l, element j of fragment (mt, ks) holding entry
+// (32 mt + l % 32, 16 ks + 8 (l / 32) + j).  az[group][m] = (cos m phi, sin m phi)
+// of the azimuth groups and off_grp the group of each offset id (-1 for none).
+void build_dense_m2l(const Tables& t, std::vector<uint16_t>& ops, int& op_exp, std::vector<float>& az,
+                     std::vector<int32_t>& off_grp);
+
+// IEEE binary16 bits of x rounded to nearest even (|x| < 65520), and back.
+uint16_t f16_bits(double x);
+double f16_value(uint16_t b);
+
 // Builds all tables for expansion order p (1 <= p <= 20).
 void build_tables(int p, Tables& t);
 

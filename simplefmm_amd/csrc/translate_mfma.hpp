@@ -11,6 +11,7 @@
 #include <type_traits>
 
 #include "kernels.hpp"
+#include "m2l_dense.hpp"
 #include "m2l_sched.hpp"
 
 namespace sfmm {
@@ -164,7 +165,11 @@ __device__ __forceinline__ void translate_wave(const TransArgs& a, int64_t blk, 
                     out[row] += acc[q][r];
                 } else {
                     out[row] = acc[q][r];
-                    if (a.out32) a.out32[(static_cast<int64_t>(dst) * NC + c) * K + row] = static_cast<float>(acc[q][r]);
+                    if (a.out32) {
+                        using FL = F32LayoutOf<P>;
+                        a.out32[(static_cast<int64_t>(dst) * NC + c) * FL::L.row + FL::L.pos[row]] =
+                            static_cast<float>(acc[q][r]);
+                    }
                 }
             }
         }

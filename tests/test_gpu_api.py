@@ -218,7 +218,8 @@ def test_failed_allocations_midway_through_the_work_lists(torch_dev, monkeypatch
 
 @pytest.mark.parametrize("p", [4, 6, 10, 16])
 def test_mixed_every_kind_against_oracle(p):
-    # the FP32 stages for every charge count (16/nc targets per M2L block) and the curl
+    # every charge count (1..4 columns per target in the M2L blocks) and the curl;
+    # p <= 12 runs the split f16 M2L, p = 16 the FP32 stages
     n = 5000
     pos, _ = W.clusters(n, seed=100 + p)
     part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, 32)

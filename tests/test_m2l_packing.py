@@ -135,3 +135,19 @@ def test_packing_covers_every_operator_entry_once(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     # p = 12: 43 + 43 + 45 MFMAs per entry instead of 52 + 52 + 52
     assert "p12 fragments 43 43 45" in r.stdout
+
+
+@pytest.mark.parametrize("p", [1, 4, 8, 12])
+def test_dense_m2l_operators_match_staged_rotations(tmp_path, p):
+    # the f16 hi/lo class operators of the mixed M2L (with their azimuth groups and
+    # the dense index order) reproduce rotate-translate-rotate applied stage by
+    # stage, for every interaction offset, to the 22 bits the split carries
+    exe = tmp_path / "dense_ops_check"
+    lib = os.path.join(ROOT, "simplefmm_amd")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(lib, "csrc"),
+                    os.path.join(ROOT, "tests", "dense_ops_check.cpp"), "-o", str(exe), "-L", lib, "-lsfmm",
+                    "-Wl,-rpath," + lib], check=True, timeout=300)
+    out = subprocess.run([str(exe), str(p)], capture_output=True, text=True, timeout=300, check=True).stdout
+    tok = out.split()
+    assert tok[0] == "max_rel" and int(tok[3]) == 316, out
+    assert float(tok[1]) < 2e-6, out
