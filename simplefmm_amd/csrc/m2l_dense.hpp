@@ -23,7 +23,9 @@
 //     split the same way, is copied into LDS by the LDS DMA one offset ahead;
 //   * at the end of an azimuth group C is rotated back by e^{-i m phi} and
 //     added into the running FP32 sum S; each target's L~ is written once in
-//     FP64 at the end (the first writer of L at levels >= 2).
+//     FP64 at the end (the first writer of L at levels >= 2).  (Rotating the
+//     accumulators from group to group instead, without S, saves registers but
+//     rounds each earlier contribution once per later group: 3x the error.)
 // Orders 1..12 (the class operator of order 12 fills 66 KiB; two of them are
 // double-buffered in LDS).
 #pragma once
@@ -38,6 +40,8 @@
 namespace sfmm {
 
 constexpr int kDenseMaxP = 12;
+constexpr int kDenseWaves = 4;  // wavefronts per workgroup (one per SIMD)
+constexpr int kDenseMaxEntries = 192;  // interaction offsets of a block (at most 189)
 constexpr int kDenseMaxSlots = 64;
 
 __host__ __device__ constexpr int dceil(int a, int b) { return (a + b - 1) / b; }
@@ -58,6 +62,8 @@ struct DenseOrder {
     int mtc = 0, mts = 0;  // 32-row tiles of the cos / sin part
     int kd = 0;            // floats per (node, component) in the FP32 copy: kc + ks
     int nfrag = 0;         // 32x16 A fragments per class
+    int chunks = 0;        // 1 KiB copy chunks per class (hi and lo of each fragment), padded so
+                           // that every wavefront of the workgroup copies the same number
     int el[2][kDenseMaxSlots] = {};  // element (h, slot): 32 n + m, -1 = padding
 };
 
@@ -99,6 +105,7 @@ __host__ __device__ constexpr DenseOrder dense_order(int p)
     o.mts = dceil(o.ks, 32);
     o.kd = o.kc + o.ks;
     o.nfrag = o.mtc * o.ksc + o.mts * o.kss;
+    o.chunks = dceil(2 * o.nfrag, kDenseWaves) * kDenseWaves;
     return o;
 }
 
@@ -211,7 +218,7 @@ __host__ __device__ constexpr int dense_row_rf(const DenseOrder& o, int part, in
 __device__ __forceinline__ void dense_split(float x, _Float16& hi, _Float16& lo)
 {
     hi = static_cast<_Float16>(x);
-    lo = static_cast<_Float16>(x - static_cast<float>(hi));
+    lo = static_cast<_Float16>(__builtin_fmaf(static_cast<float>(hi), -1.0f, x));
 }
 
 // C += A_hi B_hi + A_lo B_hi + A_hi B_lo (f16 MFMA, FP32 accumulation; the
@@ -229,8 +236,16 @@ __device__ __forceinline__ void dense_lds_copy(const dh8* src, dh8* dst)
 {
     __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
 }
-// Waits for every vector memory load of the wave, LDS DMA included.
-__device__ __forceinline__ void dense_wait_loads() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// Waits until at most N vector memory loads of the wave are in flight (LDS DMA
+// included): with the N loads issued after an operator copy, the copy is done.
+template <int N>
+__device__ __forceinline__ void dense_wait_loads()
+{
+    static_assert(N >= 0 && N <= 63, "vmcnt range");
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+// Keeps the compiler from moving memory operations across this point.
+__device__ __forceinline__ void dense_fence() { asm volatile("" ::: "memory"); }
 
 // The order of P as a static member (usable in constant expressions inside the
 // kernel's lambdas without capturing a local).
@@ -239,20 +254,62 @@ struct DenseCfg {
     static constexpr DenseOrder O = dense_order(P);
 };
 
+// Adds the accumulators of one azimuth group, rotated back by e^{-i m phi}
+// ((c, s) = az[m]), into the running sums and clears them:
+// (sc + i ss) += (lc + i ls) e^{-i m phi}.
+template <int P, int MTC, int MTS>
+__device__ __forceinline__ void dense_flush(df16* cc, df16* cs, df16* sumc, df16* sums, const float* az, int h)
+{
+    using Cfg = DenseCfg<P>;
+    dense_for<0, MTC>([&](auto mc) {
+        constexpr int mt = decltype(mc)::value;
+        dense_for<0, 16>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            constexpr int k0 = 32 * mt + 8 * (r / 4) + r % 4;
+            constexpr int m0 = dense_row_m(Cfg::O, k0, k0 + 4), m1 = dense_row_m(Cfg::O, k0 + 4, k0);
+            const float lc = cc[mt][r];
+            if constexpr (mt >= MTS || (m0 == 0 && m1 == 0)) {
+                sumc[mt][r] += lc;  // order 0 (its sin row is padding)
+                if constexpr (mt < MTS) sums[mt][r] += cs[mt][r];
+            } else {
+                float c, sn;
+                if constexpr (m0 == m1) {
+                    c = az[2 * m0];
+                    sn = az[2 * m0 + 1];
+                } else {
+                    c = h ? az[2 * m1] : az[2 * m0];
+                    sn = h ? az[2 * m1 + 1] : az[2 * m0 + 1];
+                }
+                const float ls = cs[mt][r];
+                sumc[mt][r] += lc * c + ls * sn;
+                sums[mt][r] += ls * c - lc * sn;
+            }
+        });
+        cc[mt] = df16{};
+        if constexpr (mt < MTS) cs[mt] = df16{};
+    });
+}
+
 template <int P, int NC>
-__global__ __launch_bounds__(256, 1) void m2l_dense_kernel(DenseArgs a, const float* __restrict__ M32,
-                                                            double* __restrict__ L)
+__global__ __launch_bounds__(64 * kDenseWaves, 1) void m2l_dense_kernel(DenseArgs a, const float* __restrict__ M32,
+                                                                       double* __restrict__ L)
 {
     using Cfg = DenseCfg<P>;
     constexpr int K = (P + 1) * (P + 1);
-    constexpr int NCW = NC == 3 ? 4 : NC;  // waves per target group (NC = 3: wave 3 only copies)
-    constexpr int G = 4 / NCW;             // target groups per block
+    constexpr int NW = kDenseWaves;
+    constexpr int NCW = NC == 3 ? 4 : NC;  // waves per target group (NC = 3: the fourth only copies)
+    constexpr int G = NW / NCW;            // target groups per block
     constexpr int TB = 32 * G;             // targets per block
-    constexpr int FR = Cfg::O.nfrag;
+    constexpr int CH = Cfg::O.chunks;  // LDS copy chunks of one class (64 x 16 bytes each)
     constexpr int KSC = Cfg::O.ksc, KSS = Cfg::O.kss, MTC = Cfg::O.mtc, MTS = Cfg::O.mts, KC = Cfg::O.kc,
                   KD = Cfg::O.kd;
-    // the operator of the current entry and of the next one, each [frag][hi 64 | lo 64]
-    __shared__ dh8 opbuf[2 * FR * 128];
+    static_assert(MTS <= MTC, "sin rows fit the cos tiles");
+    // the operator of the current entry and of the next one, each [frag][hi 64 | lo 64],
+    // the azimuth factors and the block's entry descriptors
+    __shared__ dh8 opbuf0[CH * 64];
+    __shared__ dh8 opbuf1[CH * 64];
+    __shared__ float saz[kAzGroups * (P + 1) * 2];
+    __shared__ int32_t smeta[kDenseMaxEntries];
 
     const int64_t per = (a.n_block + 7) / 8;
     const int64_t blk = static_cast<int64_t>(blockIdx.x % 8) * per + blockIdx.x / 8;
@@ -265,153 +322,161 @@ __global__ __launch_bounds__(256, 1) void m2l_dense_kernel(DenseArgs a, const fl
     const int64_t e0 = a.blk_row[blk];
     const int ne = static_cast<int>(a.blk_row[blk + 1] - e0);
 
-    // scale of this lane's column (target, component): the largest |value| over
-    // the target's sources maps below 2^15, so rotated values stay below
-    // 2^15.5 < 65504 and keep 22 bits down to 2^-18 of the largest
-    int sexp = 0;
+    // scale of this wavefront's columns (its component of 32 neighbouring
+    // targets): the largest |value| over their sources maps below 2^15, so
+    // rotated values stay below 2^15.5 < 65504 and keep 22 bits down to 2^-18
+    // of the largest; one value per wavefront keeps the azimuth factors scalar
+    float mx = 0.0f;
     if (active && tg >= 0) {
-        float mx = 0.0f;
-        for (int i = 0; i < ne; ++i) {
-            const int32_t sn = a.ent_src[(e0 + i) * TB + ti];
-            if (sn >= 0) mx = fmaxf(mx, a.nodemax[static_cast<int64_t>(sn) * NC + comp]);
-        }
-        if (mx > 0.0f) {
-            int ex;
-            (void)frexpf(mx, &ex);  // mx < 2^ex
-            sexp = 15 - ex;
+        for (int i0 = 0; i0 < ne; i0 += 8) {
+            int32_t sn[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) sn[u] = i0 + u < ne ? a.ent_src[(e0 + i0 + u) * TB + ti] : -1;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (sn[u] >= 0) mx = fmaxf(mx, a.nodemax[static_cast<int64_t>(sn[u]) * NC + comp]);
         }
     }
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    int sexp = 0;
+    if (mx > 0.0f) {
+        int ex;
+        (void)frexpf(mx, &ex);  // mx < 2^ex
+        sexp = 15 - ex;
+    }
+    sexp = __builtin_amdgcn_readfirstlane(sexp);
     const float sc = ldexpf(1.0f, sexp);
+    // a wave-uniform float held in a scalar register
+    auto uniform = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
 
-    // copies the operator of class cls into LDS buffer b (one 1 KiB chunk per wave instruction)
-    auto dma = [&](int cls, int b) {
-        const dh8* src = static_cast<const dh8*>(a.ops) + static_cast<int64_t>(cls) * FR * 128;
-        for (int i = w; i < 2 * FR; i += 4)
-            dense_lds_copy(src + i * 64 + lane, opbuf + b * FR * 128 + i * 64);
+    // copies the operator of class cls into LDS buffer b: CH / NW chunks of 1 KiB
+    // per wavefront, a fixed count so that the wait below can count them
+    auto dma = [&](int cls, dh8* __restrict__ buf) {
+        const dh8* src = static_cast<const dh8*>(a.ops) + static_cast<int64_t>(cls) * CH * 64;
+#pragma unroll
+        for (int c = 0; c < CH / NW; ++c)
+            dense_lds_copy(src + (w + NW * c) * 64 + lane, buf + (w + NW * c) * 64);
     };
+    for (int k = tid; k < kAzGroups * (P + 1) * 2; k += 64 * NW) saz[k] = a.az[k];
+    for (int k = tid; k < ne; k += 64 * NW) smeta[k] = a.ent_meta[e0 + k];
     auto src_ptr = [&](int32_t s) -> const float* {
-        return (active && s >= 0) ? M32 + (static_cast<int64_t>(s) * NC + comp) * KD : a.zero;
+        const float* real = M32 + (static_cast<int64_t>(s < 0 ? 0 : s) * NC + comp) * KD;
+        return (active && s >= 0) ? real : a.zero;
     };
 
+    // accumulators of the current azimuth group, and the running sums (unrotated frame)
     df16 cc[MTC], cs[MTS], sumc[MTC], sums[MTS];
     for (int i = 0; i < MTC; ++i) cc[i] = sumc[i] = df16{};
     for (int i = 0; i < MTS; ++i) cs[i] = sums[i] = df16{};
-    df8 xr[KSC], xi[KSS];
+    // the sources' columns in flight: LA K-steps ahead (K-step ks in slot ks % LA;
+    // half an entry ahead at order 12, to hold the registers)
+    constexpr int LA = (KSC % 2 == 0 && KSC >= 6) ? KSC / 2 : KSC;
+    df8 xr[LA], xi[LA];
+    const float* csp = a.zero;  // this lane's source of the current entry
+    int32_t src1 = -1;          // and of the next one (loaded an entry ahead)
 
     if (ne > 0) {
-        const int meta0 = a.ent_meta[e0];
-        dma(meta0 & 0xff, 0);
-        const float* sp = src_ptr(a.ent_src[e0 * TB + ti]);
-        dense_for<0, KSC>([&](auto kc) {
+        dma(a.ent_meta[e0] & 0xff, opbuf0);
+        csp = src_ptr(a.ent_src[e0 * TB + ti]);
+        src1 = ne > 1 ? a.ent_src[(e0 + 1) * TB + ti] : -1;
+        dense_for<0, LA>([&](auto kc) {
             constexpr int ks = decltype(kc)::value;
-            xr[ks] = *reinterpret_cast<const df8*>(sp + 16 * ks + 8 * h);
-            if constexpr (ks < KSS) xi[ks] = *reinterpret_cast<const df8*>(sp + KC + 16 * ks + 8 * h);
+            xr[ks] = *reinterpret_cast<const df8*>(csp + 16 * ks + 8 * h);
+            if constexpr (ks < KSS) xi[ks] = *reinterpret_cast<const df8*>(csp + KC + 16 * ks + 8 * h);
         });
     }
-    int grp_cur = -1;
     float csx[P + 1], csy[P + 1];  // cos, sin of m phi of the current azimuth group, times sc
-    for (int i = 0; i < ne; ++i) {
-        const int64_t e = e0 + i;
-        const int meta = a.ent_meta[e];
-        const int g = (meta >> 8) & 0xff;
-        const bool more = i + 1 < ne;
-        const int nmeta = more ? a.ent_meta[e + 1] : -1;
-        if (g != grp_cur) {
-            grp_cur = g;
-            for (int m = 0; m <= P; ++m) {
-                csx[m] = a.az[(g * (P + 1) + m) * 2] * sc;
-                csy[m] = a.az[(g * (P + 1) + m) * 2 + 1] * sc;
-            }
+    __syncthreads();  // saz, smeta
+    int i = 0, g = ne > 0 ? (smeta[0] >> 8) & 0xff : -1;
+    while (i < ne) {
+        for (int m = 0; m <= P; ++m) {
+            csx[m] = uniform(saz[(g * (P + 1) + m) * 2] * sc);
+            csy[m] = uniform(saz[(g * (P + 1) + m) * 2 + 1] * sc);
         }
-        // the operator of entry i is in LDS and every wave is done with entry i - 1
-        dense_wait_loads();
-        __syncthreads();
-        if (more) dma(nmeta & 0xff, (i + 1) & 1);
-        const float* nsp = src_ptr(more ? a.ent_src[(e + 1) * TB + ti] : -1);
-        const dh8* A = opbuf + (i & 1) * FR * 128;
-        if (active) dense_for<0, KSC>([&](auto kc) {
-            constexpr int ks = decltype(kc)::value;
-            // B fragments of this K-step: rotate by e^{i m phi}, scale, split
-            dh8 bch, bcl, bsh, bsl;
-            dense_for<0, 8>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                constexpr int s = 8 * ks + j;
-                constexpr int m0 = dense_m(Cfg::O, s, 0), m1 = dense_m(Cfg::O, s, 1);
-                float cm, sm;
-                if constexpr (m0 == m1) {
-                    cm = csx[m0];
-                    sm = csy[m0];
-                } else {
-                    cm = h ? csx[m1] : csx[m0];
-                    sm = h ? csy[m1] : csy[m0];
-                }
-                const float xrj = xr[ks][j];
-                const float xij = ks < KSS ? xi[ks < KSS ? ks : 0][j] : 0.0f;
-                const float vc = xrj * cm - xij * sm;
-                _Float16 hh, ll;
-                dense_split(vc, hh, ll);
-                bch[j] = hh;
-                bcl[j] = ll;
-                if constexpr (ks < KSS) {
-                    const float vs = xrj * sm + xij * cm;
-                    dense_split(vs, hh, ll);
-                    bsh[j] = hh;
-                    bsl[j] = ll;
-                }
-            });
-            // the next entry's columns of this K-step
-            if (more) {
-                xr[ks] = *reinterpret_cast<const df8*>(nsp + 16 * ks + 8 * h);
-                if constexpr (ks < KSS) xi[ks] = *reinterpret_cast<const df8*>(nsp + KC + 16 * ks + 8 * h);
-            }
-            dense_for<0, MTC>([&](auto mc) {
-                constexpr int mt = decltype(mc)::value;
-                constexpr int f = mt * KSC + ks;
-                const dh8 ahi = A[f * 128 + lane], alo = A[f * 128 + 64 + lane];
-                dense_mma(ahi, alo, bch, bcl, cc[mt]);
-            });
-            if constexpr (ks < KSS) {
-                dense_for<0, MTS>([&](auto mc) {
-                    constexpr int mt = decltype(mc)::value;
-                    constexpr int f = MTC * KSC + mt * KSS + ks;
-                    const dh8 ahi = A[f * 128 + lane], alo = A[f * 128 + 64 + lane];
-                    dense_mma(ahi, alo, bsh, bsl, cs[mt]);
-                });
-            }
-        });
-        // end of an azimuth group: rotate back by e^{-i m phi} into the running sum
-        if (!more || ((nmeta >> 8) & 0xff) != g) {
-            dense_for<0, MTC>([&](auto mc) {
-                constexpr int mt = decltype(mc)::value;
-                dense_for<0, 16>([&](auto rc) {
-                    constexpr int r = decltype(rc)::value;
-                    constexpr int k0 = 32 * mt + 8 * (r / 4) + r % 4;
-                    constexpr int m0 = dense_row_m(Cfg::O, k0, k0 + 4), m1 = dense_row_m(Cfg::O, k0 + 4, k0);
-                    const float* az0 = a.az + (g * (P + 1) + m0) * 2;
-                    const float* az1 = a.az + (g * (P + 1) + m1) * 2;
+        int gn = -1;  // group of the entry after this group's run
+        // one entry; even entries read opbuf0 and fill opbuf1, odd ones the reverse
+        // (two arrays, so the reads are seen not to alias the copy in flight)
+        auto entry = [&](const dh8* __restrict__ cur, dh8* __restrict__ nxt) {
+            const int64_t e = e0 + i;
+            const bool more = i + 1 < ne;
+            const int nmeta = more ? smeta[i + 1] : -1;
+            // the operator of entry i is in LDS and every wave is done with entry i - 1:
+            // after its copy (issued at the top of entry i - 1) the wave issued the
+            // source index and the refills, 2 x 16 bytes per K-step and part
+            if (i == 0) dense_wait_loads<0>();  // the prologue's copy
+            else dense_wait_loads<1 + 2 * (KSC + KSS)>();
+            __syncthreads();
+            if (more) dma(nmeta & 0xff, nxt);
+            dense_fence();
+            const float* nsp = src_ptr(src1);
+            src1 = i + 2 < ne ? a.ent_src[(e + 2) * TB + ti] : -1;
+            const dh8* A = cur;
+            if (active) dense_for<0, KSC>([&](auto kc) {
+                constexpr int ks = decltype(kc)::value;
+                // B fragments of this K-step: rotate by e^{i m phi}, scale, split
+                dh8 bch, bcl, bsh, bsl;
+                dense_for<0, 8>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    constexpr int s = 8 * ks + j;
+                    constexpr int m0 = dense_m(Cfg::O, s, 0), m1 = dense_m(Cfg::O, s, 1);
                     float cm, sm;
                     if constexpr (m0 == m1) {
-                        cm = az0[0];
-                        sm = az0[1];
+                        cm = csx[m0];
+                        sm = csy[m0];
                     } else {
-                        cm = h ? az1[0] : az0[0];
-                        sm = h ? az1[1] : az0[1];
+                        cm = h ? csx[m1] : csx[m0];
+                        sm = h ? csy[m1] : csy[m0];
                     }
-                    const float oc = cc[mt][r];
-                    if constexpr (mt < MTS) {
-                        const float os = cs[mt][r];
-                        sumc[mt][r] += oc * cm + os * sm;
-                        sums[mt][r] += os * cm - oc * sm;
-                    } else {
-                        sumc[mt][r] += oc;  // rows of order 0 only
+                    const float xrj = xr[ks % LA][j];
+                    const float xij = ks < KSS ? xi[ks % LA][j] : 0.0f;
+                    const float vc = xrj * cm - xij * sm;
+                    _Float16 hh, ll;
+                    dense_split(vc, hh, ll);
+                    bch[j] = hh;
+                    bcl[j] = ll;
+                    if constexpr (ks < KSS) {
+                        const float vs = xrj * sm + xij * cm;
+                        dense_split(vs, hh, ll);
+                        bsh[j] = hh;
+                        bsl[j] = ll;
                     }
                 });
-                cc[mt] = df16{};
-                if constexpr (mt < MTS) cs[mt] = df16{};
+                // refill this slot: K-step ks + LA of this entry, or of the next one
+                {
+                    constexpr int kn = (ks + LA) % KSC;
+                    const float* np = ks + LA < KSC ? csp : nsp;
+                    if (ks + LA < KSC || more) {
+                        xr[ks % LA] = *reinterpret_cast<const df8*>(np + 16 * kn + 8 * h);
+                        if constexpr (kn < KSS) xi[ks % LA] = *reinterpret_cast<const df8*>(np + KC + 16 * kn + 8 * h);
+                    }
+                }
+                dense_for<0, MTC>([&](auto mc) {
+                    constexpr int mt = decltype(mc)::value;
+                    constexpr int f = mt * KSC + ks;
+                    const dh8 ahi = A[f * 128 + lane], alo = A[f * 128 + 64 + lane];
+                    dense_mma(ahi, alo, bch, bcl, cc[mt]);
+                });
+                if constexpr (ks < KSS) {
+                    dense_for<0, MTS>([&](auto mc) {
+                        constexpr int mt = decltype(mc)::value;
+                        constexpr int f = MTC * KSC + mt * KSS + ks;
+                        const dh8 ahi = A[f * 128 + lane], alo = A[f * 128 + 64 + lane];
+                        dense_mma(ahi, alo, bsh, bsl, cs[mt]);
+                    });
+                }
             });
-        }
+            csp = nsp;
+            return more ? (nmeta >> 8) & 0xff : -1;
+        };
+        do {
+            gn = (i & 1) == 0 ? entry(opbuf0, opbuf1) : entry(opbuf1, opbuf0);
+            ++i;
+        } while (gn == g);
+        // the group's contribution, rotated back, into the running sums
+        if (active) dense_flush<P, MTC, MTS>(cc, cs, sumc, sums, saz + g * (P + 1) * 2, h);
+        g = gn;
     }
-    // L~ of the block's targets (first writer at its level)
+    // L~ of the block's targets (first writer at its level), unscaled, in FP64
     if (!active || tg < 0) return;
     const double un = ldexp(1.0, -a.op_exp - sexp);
     double* Lt = L + (static_cast<int64_t>(tg) * NC + comp) * K;
@@ -435,7 +500,7 @@ __global__ __launch_bounds__(256, 1) void m2l_dense_kernel(DenseArgs a, const fl
 template <int P>
 void launch_m2l_dense_p(const DenseArgs& a, int nc, const float* M32, double* L, hipStream_t s)
 {
-    const dim3 grid(static_cast<unsigned>(8 * ((a.n_block + 7) / 8))), blk(256);
+    const dim3 grid(static_cast<unsigned>(8 * ((a.n_block + 7) / 8))), blk(64 * kDenseWaves);
     switch (nc) {
     case 1: hipLaunchKernelGGL((m2l_dense_kernel<P, 1>), grid, blk, 0, s, a, M32, L); break;
     case 2: hipLaunchKernelGGL((m2l_dense_kernel<P, 2>), grid, blk, 0, s, a, M32, L); break;

@@ -421,7 +421,7 @@ void build_dense_m2l(const Tables& t, std::vector<uint16_t>& ops, int& op_exp, s
     if (dmax > 0.0) (void)std::frexp(dmax, &e);  // dmax < 2^e
     op_exp = 14 - e;
     // fragments
-    const size_t per_class = static_cast<size_t>(O.nfrag) * 1024;
+    const size_t per_class = static_cast<size_t>(O.chunks) * 512;  // padded to whole copy chunks
     ops.assign(per_class * 70, 0);
     for (int c = 0; c < 70; ++c) {
         const double* Dc = D.data() + static_cast<size_t>(c) * (O.kc * O.kc + O.ks * O.ks);

@@ -102,7 +102,7 @@ int main(int argc, char** argv)
                 Bc[dense_pos(O, rf_cos(n, m))] = xr * c - xi * s;
                 if (m) Bs[dense_pos(O, rf_sin(n, m)) - O.kc] = xr * s + xi * c;
             }
-        const size_t per_class = static_cast<size_t>(O.nfrag) * 1024;
+        const size_t per_class = static_cast<size_t>(O.chunks) * 512;
         auto entry = [&](int part, int row, int col) {
             const int mt = row / 32, ks = col / 16, l = row % 32 + 32 * ((col % 16) / 8), jj = col % 8;
             const int f = part == 0 ? mt * O.ksc + ks : O.mtc * O.ksc + mt * O.kss + ks;
