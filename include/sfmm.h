@@ -152,6 +152,21 @@ int sfmm_recorded_stage_ms(sfmm_plan plan, float* ms, int capacity, int* recorde
 int sfmm_direct(sfmm_plan plan, int kind, int ncharge, const double* Q, int64_t ntarget, const int64_t* targets,
                 double* out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): sfmm_direct with device arrays and no allocation.
+ * dQ: charges in original order as for sfmm_evaluate_device; dtargets: ntarget
+ * original indices; dout[nout*t + c] overwritten.  Asynchronous on the plan's
+ * stream; the same sums as sfmm_direct, bit for bit.  The partial sums live in
+ * a static device array of 16 MiB per GPU (held until the process ends), so
+ * ntarget <= SFMM_DIRECT_NOALLOC_MAX_TARGETS (else SFMM_ERR_ARG).  Targets are
+ * not checked on the host: one outside [0, n) gives NaN outputs.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (other streams,
+ * other threads, other plans) race and must be ordered by the caller. */
+#define SFMM_DIRECT_NOALLOC_MAX_TARGETS 524288
+int sfmm_direct_device_noalloc(sfmm_plan plan, int kind, int ncharge, const double* dQ, int64_t ntarget,
+                               const int64_t* dtargets, double* dout);
+
 /* ---- Batches of independent problems.
  * No counterpart in the reference (one problem per FMM3D object): many plans
  * (each its own tree, order and charges) evaluated by one call, spread over

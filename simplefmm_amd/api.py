@@ -27,6 +27,7 @@ LIB_PATH = os.path.join(_HERE, "libsfmm.so")
 
 POTENTIAL, CURL, POTGRAD = 0, 1, 2
 FAR, NEAR, BOTH = 1, 2, 3
+DIRECT_NOALLOC_MAX_TARGETS = 524288  # SFMM_DIRECT_NOALLOC_MAX_TARGETS
 SELF_LIST = 4  # or-ed into `which`: the near field excludes the plan's self lists
 FP64, MIXED = 0, 1
 STAGES = ("gather", "P2M", "M2M", "M2L", "L2L", "L2P", "P2P", "scatter", "total")
@@ -85,6 +86,8 @@ SIGNATURES = {
     "sfmm_recorded_stage_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_int)]),
     "sfmm_direct": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int64, _ip, _dp]),
+    "sfmm_direct_device_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                                  ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
     "sfmm_batch_create": (ctypes.c_int, [ctypes.POINTER(BatchItem), ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_void_p)]),
     "sfmm_batch_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -342,6 +345,13 @@ class FMM3D:
         out = np.zeros((len(targets), nout))
         _check(lib().sfmm_direct(self.h, kind, ncharge, _d(Q), len(targets), _i(targets), _d(out)))
         return out
+
+    def direct_device_noalloc(self, kind: int, ncharge: int, q_ptr: int, ntarget: int, t_ptr: int, out_ptr: int):
+        """The direct sum with device arrays and no device allocation (static
+        scratch; at most DIRECT_NOALLOC_MAX_TARGETS targets), asynchronous on the
+        plan's stream: out[t, c] = value at original index targets[t]."""
+        _check(lib().sfmm_direct_device_noalloc(self.h, kind, ncharge, ctypes.c_void_p(q_ptr), int(ntarget),
+                                                ctypes.c_void_p(t_ptr), ctypes.c_void_p(out_ptr)))
 
 
 class Batch:

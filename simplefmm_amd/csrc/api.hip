@@ -1296,9 +1296,7 @@ int sfmm_direct(sfmm_plan plan, int kind, int ncharge, const double* Q, int64_t 
     if (ntarget <= 0) return SFMM_OK;
     HIP_TRY(hipSetDevice(plan->device));
     if (int rc = ensure_buffers(plan, nc, qdim)) return rc;
-    const int64_t nblk = (ntarget + 63) / 64;
-    int nsplit = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(512, 8192 / nblk)));
-    nsplit = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(nsplit, n / 64 + 1)));
+    const int nsplit = sfmm::direct_nsplit(ntarget, n);
     double *dQ = nullptr, *dout = nullptr, *part = nullptr;
     int64_t* dt = nullptr;
     auto cleanup = [&]() {
@@ -1329,6 +1327,63 @@ int sfmm_direct(sfmm_plan plan, int kind, int ncharge, const double* Q, int64_t 
     return SFMM_OK;
 }
 
+
+// ------------------------------------------------------------------ direct sum without allocation
+// The partial sums of sfmm_direct_device_noalloc: ntarget * nout * nsplit
+// doubles, at most 2^19 * 4 (16 MiB) for ntarget <= 2^19.  A static device array,
+// so the call allocates nothing; every device holds its own copy, reached only
+// through the address hipGetSymbolAddress gives with that device current.
+namespace {
+
+constexpr int64_t kNoAllocScratch = 4 * int64_t(SFMM_DIRECT_NOALLOC_MAX_TARGETS);
+__device__ __align__(256) double scratch[kNoAllocScratch];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on device `dev` (current), looked up once.
+int noalloc_scratch(int dev, double** out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return fail(SFMM_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(SFMM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return SFMM_OK;
+}
+
+}  // namespace
+
+int sfmm_direct_device_noalloc(sfmm_plan plan, int kind, int ncharge, const double* dQ, int64_t ntarget,
+                               const int64_t* dtargets, double* dout)
+{
+    if (!plan || !dQ) return fail(SFMM_ERR_ARG, "null argument");
+    sfmm::Mode mode;
+    int qdim, nc;
+    if (int rc = mode_of(kind, ncharge, mode, qdim, nc)) return rc;
+    if (ntarget < 0 || ntarget > SFMM_DIRECT_NOALLOC_MAX_TARGETS)
+        return fail(SFMM_ERR_ARG, "sfmm_direct_device_noalloc: ntarget must be in 0.." +
+                                      std::to_string(SFMM_DIRECT_NOALLOC_MAX_TARGETS) + " (static scratch)");
+    if (ntarget > 0 && (!dtargets || !dout)) return fail(SFMM_ERR_ARG, "null argument");
+    if (ntarget == 0) return SFMM_OK;
+    HIP_TRY(hipSetDevice(plan->device));
+    double* part = nullptr;
+    if (int rc = noalloc_scratch(plan->device, &part)) return rc;
+    const int64_t n = plan->tree->t.n;
+    const int nsplit = sfmm::direct_nsplit(ntarget, n);
+    if (sfmm::direct_scratch_size(ntarget, nc, nsplit) > kNoAllocScratch)
+        return fail(SFMM_ERR_ARG, "sfmm_direct_device_noalloc: partial sums exceed the static scratch");
+    // every partial sum the reduction reads is written first: nothing to clear
+    sfmm::launch_direct(plan->d, mode, nc, dQ, ntarget, dtargets, dout, part, nsplit, plan->stream, plan->d.perm);
+    HIP_TRY(hipGetLastError());
+    return SFMM_OK;
+}
 
 int sfmm_batch_create(const sfmm_batch_item* items, int count, int streams, sfmm_batch* out)
 {

@@ -11,6 +11,7 @@
 #include "kernels.hpp"
 #include "m2l_dense.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace sfmm {
@@ -677,9 +678,13 @@ __global__ void p2p_reduce_kernel(DevPlan d, P2PItems w, int nout, const double*
 // ------------------------------------------------------------------ direct sum
 // FP64 O(N * ntarget) sum over all sources j != target (main.cpp:70-96), the
 // error reference.  Block (target chunk of 64, source split); partial sums are
-// reduced in a fixed order by direct_reduce_kernel.
+// reduced in a fixed order by direct_reduce_kernel.  Charges are read in slot
+// order, q[c * n + slot], or with qperm through the permutation from the
+// caller's original-order array, q[qperm[slot] * NCQ + c] (the same values in
+// the same order, so the same sums).  A target outside [0, n) gives NaN.
 template <int MODE, int NCQ, int NOUT>
-__global__ __launch_bounds__(64) void direct_kernel(DevPlan d, const double* __restrict__ q, int64_t ntarget,
+__global__ __launch_bounds__(64) void direct_kernel(DevPlan d, const double* __restrict__ q,
+                                                    const int32_t* __restrict__ qperm, int64_t ntarget,
                                                     const int64_t* __restrict__ targets, int nsplit,
                                                     double* __restrict__ partial)
 {
@@ -689,12 +694,17 @@ __global__ __launch_bounds__(64) void direct_kernel(DevPlan d, const double* __r
     const int split = blockIdx.y;
     const bool valid = k < ntarget;
     int64_t slot = -1;
+    bool bad = false;
     double xi = 0, yi = 0, zi = 0;
     if (valid) {
-        slot = d.iperm[targets[k]];
-        xi = d.px[slot];
-        yi = d.py[slot];
-        zi = d.pz[slot];
+        const int64_t t = targets[k];
+        bad = t < 0 || t >= d.n;
+        if (!bad) {
+            slot = d.iperm[t];
+            xi = d.px[slot];
+            yi = d.py[slot];
+            zi = d.pz[slot];
+        }
     }
     const int64_t j0s = d.n * split / nsplit, j1s = d.n * (split + 1) / nsplit;
     double acc[NOUT];
@@ -707,12 +717,18 @@ __global__ __launch_bounds__(64) void direct_kernel(DevPlan d, const double* __r
             sx[tid] = d.px[jl];
             sy[tid] = d.py[jl];
             sz[tid] = d.pz[jl];
+            if (qperm) {
+                const int64_t o = static_cast<int64_t>(qperm[jl]) * NCQ;
 #pragma unroll
-            for (int c = 0; c < NCQ; ++c) sq[c][tid] = q[c * d.n + jl];
+                for (int c = 0; c < NCQ; ++c) sq[c][tid] = q[o + c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < NCQ; ++c) sq[c][tid] = q[c * d.n + jl];
+            }
         }
         __syncthreads();
         const int cnt = static_cast<int>(j1s - j0 < 64 ? j1s - j0 : 64);
-        if (valid) {
+        if (valid && !bad) {
             for (int kk = 0; kk < cnt; ++kk) {
                 if (j0 + kk == slot) continue;
                 double qj[NCQ];
@@ -724,7 +740,8 @@ __global__ __launch_bounds__(64) void direct_kernel(DevPlan d, const double* __r
     }
     if (valid) {
 #pragma unroll
-        for (int c = 0; c < NOUT; ++c) partial[(static_cast<int64_t>(split) * NOUT + c) * ntarget + k] = acc[c];
+        for (int c = 0; c < NOUT; ++c)
+            partial[(static_cast<int64_t>(split) * NOUT + c) * ntarget + k] = bad ? __builtin_nan("") : acc[c];
     }
 }
 
@@ -888,25 +905,32 @@ void launch_self_correction(const DevPlan& d, const SelfCorr& c, Mode mode, int 
 
 int64_t direct_scratch_size(int64_t ntarget, int nout, int nsplit) { return ntarget * nout * nsplit; }
 
+int direct_nsplit(int64_t ntarget, int64_t n)
+{
+    const int64_t nblk = (ntarget + 63) / 64;
+    const int64_t ns = std::max<int64_t>(1, std::min<int64_t>(512, 8192 / std::max<int64_t>(nblk, 1)));
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ns, n / 64 + 1)));
+}
+
 void launch_direct(const DevPlan& d, Mode mode, int nc, const double* q, int64_t ntarget, const int64_t* targets,
-                   double* out, double* partial, int nsplit, hipStream_t s)
+                   double* out, double* partial, int nsplit, hipStream_t s, const int32_t* qperm)
 {
     if (ntarget == 0) return;
     const dim3 grid(grid_for(ntarget, 64), nsplit), blk(64);
     int nout = nc;
     if (mode == MODE_POTGRAD) {
-        hipLaunchKernelGGL((direct_kernel<MODE_POTGRAD, 1, 4>), grid, blk, 0, s, d, q, ntarget, targets, nsplit,
+        hipLaunchKernelGGL((direct_kernel<MODE_POTGRAD, 1, 4>), grid, blk, 0, s, d, q, qperm, ntarget, targets, nsplit,
                            partial);
         nout = 4;
     } else if (mode == MODE_CURL) {
-        hipLaunchKernelGGL((direct_kernel<MODE_CURL, 3, 3>), grid, blk, 0, s, d, q, ntarget, targets, nsplit, partial);
+        hipLaunchKernelGGL((direct_kernel<MODE_CURL, 3, 3>), grid, blk, 0, s, d, q, qperm, ntarget, targets, nsplit, partial);
         nout = 3;
     } else {
         switch (nc) {
-        case 1: hipLaunchKernelGGL((direct_kernel<MODE_POT, 1, 1>), grid, blk, 0, s, d, q, ntarget, targets, nsplit, partial); break;
-        case 2: hipLaunchKernelGGL((direct_kernel<MODE_POT, 2, 2>), grid, blk, 0, s, d, q, ntarget, targets, nsplit, partial); break;
-        case 3: hipLaunchKernelGGL((direct_kernel<MODE_POT, 3, 3>), grid, blk, 0, s, d, q, ntarget, targets, nsplit, partial); break;
-        default: hipLaunchKernelGGL((direct_kernel<MODE_POT, 4, 4>), grid, blk, 0, s, d, q, ntarget, targets, nsplit, partial); break;
+        case 1: hipLaunchKernelGGL((direct_kernel<MODE_POT, 1, 1>), grid, blk, 0, s, d, q, qperm, ntarget, targets, nsplit, partial); break;
+        case 2: hipLaunchKernelGGL((direct_kernel<MODE_POT, 2, 2>), grid, blk, 0, s, d, q, qperm, ntarget, targets, nsplit, partial); break;
+        case 3: hipLaunchKernelGGL((direct_kernel<MODE_POT, 3, 3>), grid, blk, 0, s, d, q, qperm, ntarget, targets, nsplit, partial); break;
+        default: hipLaunchKernelGGL((direct_kernel<MODE_POT, 4, 4>), grid, blk, 0, s, d, q, qperm, ntarget, targets, nsplit, partial); break;
         }
     }
     hipLaunchKernelGGL(direct_reduce_kernel, dim3(grid_for(ntarget, 256)), dim3(256), 0, s, ntarget, nout, nsplit,

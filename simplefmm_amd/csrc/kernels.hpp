@@ -157,11 +157,17 @@ void launch_p2p(const DevPlan& d, const P2PItems& w, Mode mode, int nc, const do
 // use_near (the near field then used the lists as its exclusion predicate).
 void launch_self_correction(const DevPlan& d, const SelfCorr& c, Mode mode, int nc, bool use_near, const double* q,
                             double* out, hipStream_t s);
+// q: charges in slot order ([c][slot]), or, with qperm = the slot -> original
+// index map, the caller's original-order array ([index][c]).  Targets outside
+// [0, n) give NaN.
 void launch_direct(const DevPlan& d, Mode mode, int nc, const double* q, int64_t ntarget, const int64_t* targets,
-                   double* out, double* partial, int nsplit, hipStream_t s);
+                   double* out, double* partial, int nsplit, hipStream_t s, const int32_t* qperm = nullptr);
 void launch_fill(double* x, int64_t count, double v, hipStream_t s);
 
 // Number of double slots the direct sum needs as scratch for `ntarget` targets.
 int64_t direct_scratch_size(int64_t ntarget, int nout, int nsplit);
+// Source splits of the direct sum for `ntarget` targets among n particles:
+// ntarget * nsplit <= 2^19 whenever ntarget <= 2^19.
+int direct_nsplit(int64_t ntarget, int64_t n);
 
 }  // namespace sfmm
