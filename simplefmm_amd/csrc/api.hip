@@ -379,8 +379,9 @@ int cut_m2l_blocks(sfmm_plan_s* P, int TB, const std::vector<int>& order, const 
 int build_dense_blocks(sfmm_plan_s* P, int nc)
 {
     if (P->dns[nc].blk_row) return SFMM_OK;
+    // targets per block: 16 per group of waves, one wave per component
     const int ncw = nc == 3 ? 4 : nc;
-    const int TB = 32 * (4 / ncw);
+    const int TB = 16 * (sfmm::kDenseWaves / ncw);
     const auto& cls = P->tables.off_class;
     const auto& grp = P->dense_grp;
     std::vector<int> order;

@@ -381,23 +381,24 @@ void build_dense_m2l(const Tables& t, std::vector<uint16_t>& ops, int& op_exp, s
         if (t.off_class[id] >= 0 && std::max({std::abs(di), std::abs(dj), std::abs(dk)}) >= 2)
             cls_d2[t.off_class[id]] = t.off_d2[id];
     }
-    // dense blocks per class: part 0 cos [kc][kc], part 1 sin [ks][ks]
-    std::vector<double> D(static_cast<size_t>(70) * (O.kc * O.kc + O.ks * O.ks), 0.0);
+    // dense blocks per class and part, [row][column] in the rows' and the
+    // columns' order (16 mtc rows by kc columns each)
+    const int nrow = 16 * O.mtc, ncol = O.kc;
+    const size_t per_part = static_cast<size_t>(nrow) * ncol;
+    std::vector<double> D(70 * 2 * per_part, 0.0);
     double dmax = 0.0;
     for (int c = 0; c < 70; ++c) {
         if (cls_d2[c] < 1) continue;
         const double* fwd = t.rot.data() + static_cast<size_t>(c) * 2 * t.rot_size;
         const double* bwd = fwd + t.rot_size;
         const double* C = t.coax.data() + static_cast<size_t>(cls_d2[c]) * t.coax_size;
-        double* Dc = D.data() + static_cast<size_t>(c) * (O.kc * O.kc + O.ks * O.ks);
         for (int part = 0; part < 2; ++part) {
-            const int ext = part == 0 ? O.kc : O.ks;
-            double* Dp = Dc + (part == 0 ? 0 : O.kc * O.kc);
-            for (int kr = 0; kr < ext; ++kr) {
-                const int er = dense_elem(O, part, kr);
+            double* Dp = D.data() + (static_cast<size_t>(c) * 2 + part) * per_part;
+            for (int kr = 0; kr < nrow; ++kr) {
+                const int er = dense_row_elem(O, part, kr);
                 if (er < 0) continue;
                 const int j = er / 32, a = er % 32;
-                for (int kk = 0; kk < ext; ++kk) {
+                for (int kk = 0; kk < ncol; ++kk) {
                     const int ec = dense_elem(O, part, kk);
                     if (ec < 0) continue;
                     const int n = ec / 32, b = ec % 32;
@@ -411,7 +412,7 @@ void build_dense_m2l(const Tables& t, std::vector<uint16_t>& ops, int& op_exp, s
                             v += bwd[t.rot_off[j] + (j + 1) * (j + 1) + (a - 1) * j + (q - 1)] * tq *
                                  fwd[t.rot_off[n] + (n + 1) * (n + 1) + (q - 1) * n + (b - 1)];
                     }
-                    Dp[kr * ext + kk] = v;
+                    Dp[static_cast<size_t>(kr) * ncol + kk] = v;
                     dmax = std::max(dmax, std::fabs(v));
                 }
             }
@@ -420,30 +421,30 @@ void build_dense_m2l(const Tables& t, std::vector<uint16_t>& ops, int& op_exp, s
     int e = 0;
     if (dmax > 0.0) (void)std::frexp(dmax, &e);  // dmax < 2^e
     op_exp = 14 - e;
-    // fragments
+    // fragments: (part, 16-row tile mt, 32-wide K-step ks), lane l element jj =
+    // A[16 mt + l % 16][32 ks + 8 (l / 16) + jj] (the A operand of a 16x16x32 MFMA);
+    // cos fragment mt * nks + ks, sin fragment (mtc + mt) * nks + ks
     const size_t per_class = static_cast<size_t>(O.chunks) * 512;  // padded to whole copy chunks
     ops.assign(per_class * 70, 0);
-    for (int c = 0; c < 70; ++c) {
-        const double* Dc = D.data() + static_cast<size_t>(c) * (O.kc * O.kc + O.ks * O.ks);
+    for (int c = 0; c < 70; ++c)
         for (int part = 0; part < 2; ++part) {
-            const int ext = part == 0 ? O.kc : O.ks, mts = part == 0 ? O.mtc : O.mts, kss = part == 0 ? O.ksc : O.kss;
-            const double* Dp = Dc + (part == 0 ? 0 : O.kc * O.kc);
+            const double* Dp = D.data() + (static_cast<size_t>(c) * 2 + part) * per_part;
+            const int mts = part == 0 ? O.mtc : O.mts;
             for (int mt = 0; mt < mts; ++mt)
-                for (int ks = 0; ks < kss; ++ks) {
-                    const int f = part == 0 ? mt * O.ksc + ks : O.mtc * O.ksc + mt * O.kss + ks;
+                for (int ks = 0; ks < O.nks; ++ks) {
+                    const int f = (part == 0 ? mt : O.mtc + mt) * O.nks + ks;
                     uint16_t* hi = ops.data() + per_class * c + static_cast<size_t>(f) * 1024;
                     uint16_t* lo = hi + 512;
                     for (int l = 0; l < 64; ++l)
                         for (int jj = 0; jj < 8; ++jj) {
-                            const int row = 32 * mt + l % 32, col = 16 * ks + 8 * (l / 32) + jj;
-                            const double x = (row < ext && col < ext) ? std::ldexp(Dp[row * ext + col], op_exp) : 0.0;
+                            const int row = 16 * mt + l % 16, col = 32 * ks + 8 * (l / 16) + jj;
+                            const double x = std::ldexp(Dp[static_cast<size_t>(row) * ncol + col], op_exp);
                             const uint16_t bh = f16_bits(x);
                             hi[l * 8 + jj] = bh;
                             lo[l * 8 + jj] = f16_bits(x - f16_value(bh));
                         }
                 }
         }
-    }
     // azimuth groups: the primitive directions (a, b) of (di, dj), (0, 0) with (1, 0)
     std::vector<std::pair<int, int>> dirs;
     off_grp.assign(343, -1);

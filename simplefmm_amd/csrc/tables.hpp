@@ -59,9 +59,9 @@ void build_m2l_fragments(const Tables& t, bool interleave_rows, std::vector<doub
 // (back rotation, coaxial translation, forward rotation) in its cos and sin
 // blocks, rows and columns in dense_order(p), scaled by 2^op_exp (largest entry
 // below 2^14) and split as hi = f16(x), lo = f16(x - hi); stored as the A
-// fragments of 32x32x16 MFMA tiles, ops[class][fragment][hi, lo][64 lanes][8]
+// fragments of 16x16x32 MFMA tiles, ops[class][fragment][hi, lo][64 lanes][8]
 // with lane l, element j of fragment (mt, ks) holding entry
-// (32 mt + l % 32, 16 ks + 8 (l / 32) + j).  az[group][m] = (cos m phi, sin m phi)
+// (16 mt + l % 16, 32 ks + 8 (l / 16) + j).  az[group][m] = (cos m phi, sin m phi)
 // of the azimuth groups and off_grp the group of each offset id (-1 for none).
 void build_dense_m2l(const Tables& t, std::vector<uint16_t>& ops, int& op_exp, std::vector<float>& az,
                      std::vector<int32_t>& off_grp);

@@ -86,42 +86,56 @@ int main(int argc, char** argv)
                 Lref[rf_cos(j, a)] = lc * c + ls * s;
                 if (a > 0) Lref[rf_sin(j, a)] = ls * c - lc * s;
             }
-        // dense: the same azimuth (from the FP32 group table), operator decoded from the fragments
+        // dense, as the kernel computes it: the multipole in the FP32 copy's order,
+        // columns rotated with the slot's azimuth factor (1 for order 0), the
+        // operator decoded from the fragments by the MFMA lane layout, rows rotated
+        // back by their order and written to their real-form index
         const int g = grp[id];
         if (g < 0) { std::printf("offset %d without an azimuth group\n", id); return 1; }
         const float* ag = az.data() + static_cast<size_t>(g) * (p + 1) * 2;
-        std::vector<double> Bc(O.kc, 0.0), Bs(O.ks, 0.0);
+        std::vector<double> X32(O.kd, 0.0);
         for (int r = 0; r < K; ++r) {
             const int pos = dense_pos(O, r);
-            if (pos < 0) { std::printf("coefficient %d has no position\n", r); return 1; }
+            if (pos < 0 || pos >= O.kd) { std::printf("coefficient %d has no position\n", r); return 1; }
+            X32[pos] = M[r];
         }
-        for (int n = 0; n <= p; ++n)
-            for (int m = 0; m <= n; ++m) {
-                const double xr = M[rf_cos(n, m)], xi = m ? M[rf_sin(n, m)] : 0.0;
-                const double c = ag[2 * m], s = ag[2 * m + 1];
-                Bc[dense_pos(O, rf_cos(n, m))] = xr * c - xi * s;
-                if (m) Bs[dense_pos(O, rf_sin(n, m)) - O.kc] = xr * s + xi * c;
-            }
+        std::vector<double> Bc(O.kc, 0.0), Bs(O.kc, 0.0);
+        for (int k = 0; k < O.kc; ++k) {
+            const int s = dense_kslot(k), q = dense_kq(k);
+            const int m = dense_slot_m(O, s);
+            const bool zero = (dense_slot_zero(O, s) >> q) & 1;
+            const double c = zero ? 1.0 : ag[2 * m], sn = zero ? 0.0 : ag[2 * m + 1];
+            Bc[k] = X32[k] * c - X32[O.kc + k] * sn;
+            Bs[k] = X32[k] * sn + X32[O.kc + k] * c;
+        }
         const size_t per_class = static_cast<size_t>(O.chunks) * 512;
         auto entry = [&](int part, int row, int col) {
-            const int mt = row / 32, ks = col / 16, l = row % 32 + 32 * ((col % 16) / 8), jj = col % 8;
-            const int f = part == 0 ? mt * O.ksc + ks : O.mtc * O.ksc + mt * O.kss + ks;
+            const int mt = row / 16, ks = col / 32, l = row % 16 + 16 * ((col % 32) / 8), jj = col % 8;
+            const int f = (part == 0 ? mt : O.mtc + mt) * O.nks + ks;
             const uint16_t* hi = ops.data() + per_class * cls + static_cast<size_t>(f) * 1024;
             return std::ldexp(f16_value(hi[l * 8 + jj]) + f16_value(hi[512 + l * 8 + jj]), -op_exp);
         };
-        std::vector<double> Lc(O.kc, 0.0), Ls(O.ks, 0.0);
-        for (int r = 0; r < O.kc; ++r)
-            for (int c = 0; c < O.kc; ++c) Lc[r] += entry(0, r, c) * Bc[c];
-        for (int r = 0; r < O.ks; ++r)
-            for (int c = 0; c < O.ks; ++c) Ls[r] += entry(1, r, c) * Bs[c];
         std::vector<double> Ld(K, 0.0);
-        for (int j = 0; j <= p; ++j)
-            for (int a = 0; a <= j; ++a) {
-                const double lc = Lc[dense_pos(O, rf_cos(j, a))], ls = a ? Ls[dense_pos(O, rf_sin(j, a)) - O.kc] : 0.0;
-                const double c = ag[2 * a], s = ag[2 * a + 1];
-                Ld[rf_cos(j, a)] = lc * c + ls * s;
-                if (a > 0) Ld[rf_sin(j, a)] = ls * c - lc * s;
+        std::vector<int> seen(K, 0);
+        for (int row = 0; row < 16 * O.mtc; ++row) {
+            double lc = 0, ls = 0;
+            for (int c = 0; c < O.kc; ++c) lc += entry(0, row, c) * Bc[c];
+            if (row < 16 * O.mts)
+                for (int c = 0; c < O.kc; ++c) ls += entry(1, row, c) * Bs[c];
+            const int ec = dense_row_elem(O, 0, row), es = dense_row_elem(O, 1, row);
+            const int m = ec < 0 ? 0 : ec % 32;
+            const double c = ag[2 * m], sn = ag[2 * m + 1];
+            if (ec >= 0) {
+                Ld[dense_rf(ec, 0)] = lc * c + ls * sn;
+                ++seen[dense_rf(ec, 0)];
             }
+            if (es >= 0) {
+                Ld[dense_rf(es, 1)] = ls * c - lc * sn;
+                ++seen[dense_rf(es, 1)];
+            }
+        }
+        for (int r = 0; r < K; ++r)
+            if (seen[r] != 1) { std::printf("coefficient %d written %d times\n", r, seen[r]); return 1; }
         double num = 0, den = 0;
         for (int r = 0; r < K; ++r) {
             num += (Ld[r] - Lref[r]) * (Ld[r] - Lref[r]);
