@@ -29,6 +29,7 @@ from simplefmm_amd import api, workloads as W  # noqa: E402
 
 PEAK_FP64 = 78.6e12    # MI355X dense FP64 (vector and MFMA), FLOP/s
 PEAK_FP32 = 157.3e12   # MI355X dense FP32 MFMA, FLOP/s
+PEAK_F16 = 2.5e15      # MI355X dense FP16 MFMA, FLOP/s
 PEAK_HBM = 8.0e12      # bytes/s
 
 WORKLOADS = {
@@ -61,6 +62,16 @@ def m2l_flops_real(p: int) -> int:
     mac = sum((n + 1) ** 2 + n * n for n in range(p + 1))
     azi = 4 * p * (p + 1) // 2
     return 2 * (3 * mac + 2 * azi)
+
+
+def m2l_flops_split(p: int) -> int:
+    """Per (pair, component) flops of the mixed path's M2L as it runs on the f16
+    matrix cores (p <= 12): the dense real-form class operator, a cos block of
+    (p+1)(p+2)/2 and a sin block of p(p+1)/2 rows and columns, applied as three
+    split products (hi x hi, lo x hi, hi x lo), 2 flops per multiply-add."""
+    nc = (p + 1) * (p + 2) // 2
+    ns = p * (p + 1) // 2
+    return 3 * 2 * (nc * nc + ns * ns)
 
 
 def cpu_baseline(gen: str, p: int, cap: int, sample_n: int, seed: int):
@@ -297,9 +308,18 @@ def main() -> int:
         err_grad = rel(U[tg, 1:], ref[:, 1:])
         log(f"[bench] {prec}: rel L2 vs direct on {len(tg)} targets: phi {err_phi:.3e} grad {err_grad:.3e}")
         m2l_ms = mean["M2L"]
-        peak = PEAK_FP32 if prec == "mixed" else PEAK_FP64
+        # the M2L kernel's matrix-core precision: f16 (split products) for the mixed
+        # path at p <= 12, FP32 above, FP64 on the plain path
+        if prec == "mixed":
+            split = p <= 12
+            peak, kernel = (PEAK_F16, "m2l_dense_kernel (f16 split)") if split else (PEAK_FP32, "m2l_pipe_kernel (fp32)")
+        else:
+            peak, kernel = PEAK_FP64, "m2l_pipe_kernel (fp64)"
+            split = False
+        # algorithmic flops: the complex full-m count of rotate-translate-rotate per
+        # (pair, component); executed: what the kernel's arithmetic needs without padding
         flops_complex = info.m2l_pairs * 4 * m2l_flops_complex(p)
-        flops_real = info.m2l_pairs * 4 * m2l_flops_real(p)
+        flops_exec = info.m2l_pairs * 4 * (m2l_flops_split(p) if split else m2l_flops_real(p))
         achieved = flops_complex / (m2l_ms * 1e-3)
         del fmm
         return {
@@ -307,11 +327,12 @@ def main() -> int:
             "error": {"targets": int(len(tg)), "rel_l2_phi": err_phi, "rel_l2_grad": err_grad,
                       "within_1e-6": bool(max(err_phi, err_grad) <= 1e-6)},
             "stages_ms": mean,
-            "roofline": {"kernel": "m2l", "bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
+            "roofline": {"kernel": kernel, "bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                          "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                          "flops_per_launch": flops_complex, "avg_launch_ms": m2l_ms,
-                         "executed_flops_per_launch": flops_real,
-                         "executed_frac": flops_real / (m2l_ms * 1e-3) / peak},
+                         "executed_flops_per_launch": flops_exec,
+                         "executed_tflops": flops_exec / (m2l_ms * 1e-3) / 1e12,
+                         "executed_frac": flops_exec / (m2l_ms * 1e-3) / peak},
         }
 
     paths = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
@@ -331,7 +352,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if "mixed" not in res else "f64+f32",
+        "dtype": "f64" if "mixed" not in res else "f64+f32+f16",
         "data": "synthetic",
         "precision_path": "mixed" if "mixed" in res else "fp64",
         "config": {"workload": args.workload, "generator": gen, "n": n, "p": p, "leaf_capacity": cap,

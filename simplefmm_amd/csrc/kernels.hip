@@ -499,6 +499,41 @@ __device__ __forceinline__ void pair_kernel(double dx, double dy, double dz, con
     }
 }
 
+// 1/sqrt(r2) in FP64 from the hardware estimate (v_rsq_f64, relative error
+// below 6e-8 on gfx950) and one Newton step y <- y + y e / 2, e = 1 - r2 y^2,
+// which squares it: below 5e-15 (measured 4.3e-15 over 2^22 values spanning
+// 2^-58..2^58), a few ulp of the correctly rounded value.
+__device__ __forceinline__ double rsqrt_nr(double r2)
+{
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double e = __builtin_fma(-r2 * y, y, 1.0);
+    return __builtin_fma(0.5 * y, e, y);
+}
+
+// pair_kernel with rsqrt_nr (the near field's FP64 pairs).
+template <int MODE, int NCQ>
+__device__ __forceinline__ void pair_kernel_nr(double dx, double dy, double dz, const double* qj, double* acc)
+{
+    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+    const double rinv = rsqrt_nr(r2);
+    if constexpr (MODE == MODE_POT) {
+#pragma unroll
+        for (int c = 0; c < NCQ; ++c) acc[c] = __builtin_fma(qj[c], rinv, acc[c]);
+    } else if constexpr (MODE == MODE_CURL) {
+        const double r3 = rinv * rinv * rinv;
+        acc[0] += (qj[1] * dz - qj[2] * dy) * r3;
+        acc[1] += (qj[2] * dx - qj[0] * dz) * r3;
+        acc[2] += (qj[0] * dy - qj[1] * dx) * r3;
+    } else {
+        const double qr = qj[0] * rinv;
+        const double qr3 = qr * rinv * rinv;
+        acc[0] += qr;
+        acc[1] = __builtin_fma(-qr3, dx, acc[1]);
+        acc[2] = __builtin_fma(-qr3, dy, acc[2]);
+        acc[3] = __builtin_fma(-qr3, dz, acc[3]);
+    }
+}
+
 // Mixed-precision near field: the pair's coordinate differences arrive in
 // FP32 (taken in FP64), the pair arithmetic is FP32 (v_rsq_f32), and each
 // target's sum over a source tile of 64 is added to an FP64 accumulator.
@@ -529,8 +564,10 @@ __device__ __forceinline__ void pair_kernel_f32(float dx, float dy, float dz, co
 // a target leaf and a run of at most a few thousand sources of its near list
 // (the concatenation of its merged source ranges): leaves whose near list is
 // long (coarse leaves next to deep subtrees) are split over several items so
-// that no wavefront carries a long tail.  Lanes are (target, split): a leaf of
-// nt < 64 targets spreads each target over floor(64/nt) lanes.  Sources are
+// that no wavefront carries a long tail.  Each lane holds two targets (so every
+// source read from LDS serves two pairs, and the two pair computations are
+// independent); the U = ceil(nt / 2) target pairs of a leaf are spread over
+// S = floor(64 / U) lanes each, which take every S-th source.  Sources are
 // staged through LDS 64 at a time; only tiles that hold particles of the
 // target leaf itself test for the pair (i, i) (the reference's default
 // is_self, fmm.hpp:875-881).  An unsplit item adds into the output; the items
@@ -547,7 +584,7 @@ __global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, P2PItems w, const do
     __shared__ double sx[64], sy[64], sz[64];
     __shared__ R sq[NCQ][64];
     __shared__ int32_t sidx[64];
-    __shared__ double red[NOUT][64];
+    __shared__ double red[2][NOUT][64];
     const int64_t it = blockIdx.x;
     const int tid = threadIdx.x;
     const int64_t g = w.leaf[it];
@@ -556,78 +593,132 @@ __global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, P2PItems w, const do
     const int64_t tb = d.lf_begin[g], te = d.lf_end[g];
     const int64_t nt = te - tb;
     const int64_t r0 = d.near_row[g], r1 = d.near_row[g + 1];
-    const int ntl = nt < 64 ? static_cast<int>(nt) : 64;  // targets per pass
-    const int S = 64 / ntl;
-    const int t = tid % ntl, s = tid / ntl;
+    const int ntp = nt < 128 ? static_cast<int>(nt) : 128;  // targets per pass
+    const int U = (ntp + 1) / 2;                              // lanes' target pairs per pass
+    const int S = 64 / U;
+    const int t = tid % U, s = tid / U;
     const bool lane_on = s < S;
-    for (int64_t t0 = tb; t0 < te; t0 += ntl) {
-        const int64_t slot = t0 + t;
-        const bool valid = lane_on && slot < te;
-        double xi = 0, yi = 0, zi = 0;
-        if (valid) {
-            xi = d.px[slot];
-            yi = d.py[slot];
-            zi = d.pz[slot];
+    for (int64_t t0 = tb; t0 < te; t0 += ntp) {
+        // this lane's targets: t0 + t and t0 + U + t
+        const int64_t slot_a = t0 + t, slot_b = t0 + U + t;
+        const bool va = lane_on && slot_a < te, vb = lane_on && slot_b < te;
+        double xa = 0, ya = 0, za = 0, xb = 0, yb = 0, zb = 0;
+        if (va) {
+            xa = d.px[slot_a];
+            ya = d.py[slot_a];
+            za = d.pz[slot_a];
         }
-        const int32_t slot32 = static_cast<int32_t>(slot);
-        double acc[NOUT];
+        if (vb) {
+            xb = d.px[slot_b];
+            yb = d.py[slot_b];
+            zb = d.pz[slot_b];
+        }
+        // the targets' positions are waited for here, not inside the pair loop, where
+        // in-order load counting would also wait for the next tile's prefetch
+        asm volatile("" ::"v"(xa), "v"(ya), "v"(za), "v"(xb), "v"(yb), "v"(zb));
+        const int32_t ia = static_cast<int32_t>(slot_a), ib = static_cast<int32_t>(slot_b);
+        double acc_a[NOUT], acc_b[NOUT];
 #pragma unroll
-        for (int c = 0; c < NOUT; ++c) acc[c] = 0.0;
+        for (int c = 0; c < NOUT; ++c) acc_a[c] = acc_b[c] = 0.0;
         int64_t rc = r0;  // range holding source u of this lane, advanced monotonically
-        for (int64_t ut = u0; ut < u1; ut += 64) {
-            __syncthreads();
+        // this lane's source of a tile, loaded one tile ahead of its use
+        double fx = 0, fy = 0, fz = 0;
+        R fq[NCQ];
+        int64_t fj = -1;
+        auto fetch = [&](int64_t ut) {
             const int64_t u = ut + tid;
-            bool mine = false;
+            fj = -1;
             if (u < u1) {
                 while (rc + 1 < r1 && d.near_pref[rc + 1] <= u) ++rc;
-                const int64_t j = d.near_begin[rc] + (u - d.near_pref[rc]);
-                mine = j >= tb && j < te;
-                sidx[tid] = static_cast<int32_t>(j);
-                sx[tid] = d.px[j];
-                sy[tid] = d.py[j];
-                sz[tid] = d.pz[j];
+                fj = d.near_begin[rc] + (u - d.near_pref[rc]);
+                fx = d.px[fj];
+                fy = d.py[fj];
+                fz = d.pz[fj];
 #pragma unroll
-                for (int c = 0; c < NCQ; ++c) sq[c][tid] = static_cast<R>(q[c * d.n + j]);
+                for (int c = 0; c < NCQ; ++c) fq[c] = static_cast<R>(q[c * d.n + fj]);
+            }
+        };
+        if (u0 < u1) fetch(u0);
+        for (int64_t ut = u0; ut < u1; ut += 64) {
+            __syncthreads();
+            const bool mine = fj >= tb && fj < te;
+            if (fj >= 0) {
+                sidx[tid] = static_cast<int32_t>(fj);
+                sx[tid] = fx;
+                sy[tid] = fy;
+                sz[tid] = fz;
+#pragma unroll
+                for (int c = 0; c < NCQ; ++c) sq[c][tid] = fq[c];
             }
             const bool own = __any(mine);  // the tile holds particles of the target leaf
             __syncthreads();
+            if (ut + 64 < u1) fetch(ut + 64);
             const int cnt = static_cast<int>(u1 - ut < 64 ? u1 - ut : 64);
-            if (valid) {
-                R a[NOUT];
+            if (va) {
+                R a[NOUT], b[NOUT];
 #pragma unroll
-                for (int c = 0; c < NOUT; ++c) a[c] = 0;
+                for (int c = 0; c < NOUT; ++c) a[c] = b[c] = 0;
                 auto pairs = [&](auto checked) {
                     for (int k = s; k < cnt; k += S) {
-                        if constexpr (decltype(checked)::value)
-                            if (sidx[k] == slot32) continue;
-                        R qj[NCQ];
+                        R qa[NCQ], qb[NCQ];
 #pragma unroll
-                        for (int c = 0; c < NCQ; ++c) qj[c] = sq[c][k];
-                        const double dx = xi - sx[k], dy = yi - sy[k], dz = zi - sz[k];
-                        if constexpr (MIXED)
-                            pair_kernel_f32<MODE, NCQ>(static_cast<float>(dx), static_cast<float>(dy),
-                                                       static_cast<float>(dz), qj, a);
-                        else
-                            pair_kernel<MODE, NCQ>(dx, dy, dz, qj, a);
+                        for (int c = 0; c < NCQ; ++c) qa[c] = qb[c] = sq[c][k];
+                        const double xj = sx[k], yj = sy[k], zj = sz[k];
+                        double dxa = xa - xj, dya = ya - yj, dza = za - zj;
+                        double dxb = xb - xj, dyb = yb - yj, dzb = zb - zj;
+                        if constexpr (decltype(checked)::value) {
+                            // the pair (i, i) contributes nothing: unit distance, zero charge
+                            const int32_t sj = sidx[k];
+                            if (sj == ia) {
+                                dxa = 1.0;
+#pragma unroll
+                                for (int c = 0; c < NCQ; ++c) qa[c] = 0;
+                            }
+                            if (sj == ib) {
+                                dxb = 1.0;
+#pragma unroll
+                                for (int c = 0; c < NCQ; ++c) qb[c] = 0;
+                            }
+                        }
+                        if constexpr (MIXED) {
+                            pair_kernel_f32<MODE, NCQ>(static_cast<float>(dxa), static_cast<float>(dya),
+                                                       static_cast<float>(dza), qa, a);
+                            pair_kernel_f32<MODE, NCQ>(static_cast<float>(dxb), static_cast<float>(dyb),
+                                                       static_cast<float>(dzb), qb, b);
+                        } else {
+                            pair_kernel_nr<MODE, NCQ>(dxa, dya, dza, qa, a);
+                            pair_kernel_nr<MODE, NCQ>(dxb, dyb, dzb, qb, b);
+                        }
                     }
                 };
                 if (own) pairs(std::true_type{});
                 else pairs(std::false_type{});
 #pragma unroll
-                for (int c = 0; c < NOUT; ++c) acc[c] += static_cast<double>(a[c]);
+                for (int c = 0; c < NOUT; ++c) {
+                    acc_a[c] += static_cast<double>(a[c]);
+                    acc_b[c] += static_cast<double>(b[c]);
+                }
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int c = 0; c < NOUT; ++c) red[c][tid] = acc[c];
+        for (int c = 0; c < NOUT; ++c) {
+            red[0][c][tid] = acc_a[c];
+            red[1][c][tid] = acc_b[c];
+        }
         __syncthreads();
-        if (s == 0 && valid) {
+        if (s == 0) {
 #pragma unroll
-            for (int c = 0; c < NOUT; ++c) {
-                double v = 0.0;
-                for (int ss = 0; ss < S; ++ss) v += red[c][t + ss * ntl];
-                if (pslot < 0) out[c * d.n + slot] += v;
-                else partial[(pslot + (slot - tb)) * NOUT + c] = v;
+            for (int h = 0; h < 2; ++h) {
+                const int64_t slot = h == 0 ? slot_a : slot_b;
+                if (slot >= te) continue;
+#pragma unroll
+                for (int c = 0; c < NOUT; ++c) {
+                    double v = 0.0;
+                    for (int ss = 0; ss < S; ++ss) v += red[h][c][t + ss * U];
+                    if (pslot < 0) out[c * d.n + slot] += v;
+                    else partial[(pslot + (slot - tb)) * NOUT + c] = v;
+                }
             }
         }
     }

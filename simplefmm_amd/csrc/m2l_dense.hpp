@@ -223,10 +223,20 @@ __device__ __forceinline__ void dense_for(F&& f)
 }
 
 // x = hi + lo in f16 to 22 bits (for |x| >= 2^-3; below, to 2^-25 absolute).
-__device__ __forceinline__ void dense_split(float x, _Float16& hi, _Float16& lo)
+// neg1 = -1 held where the compiler cannot see it: lo = f16(hi * neg1 + x) then
+// stays one mixed-precision FMA (f16 operand, f32 operand, f16 result) instead
+// of a conversion, a subtraction and a conversion.
+__device__ __forceinline__ void dense_split(float x, _Float16& hi, _Float16& lo, float neg1)
 {
     hi = static_cast<_Float16>(x);
-    lo = static_cast<_Float16>(__builtin_fmaf(static_cast<float>(hi), -1.0f, x));
+    lo = static_cast<_Float16>(__builtin_fmaf(static_cast<float>(hi), neg1, x));
+}
+// -1.0f in a scalar register, opaque to constant folding.
+__device__ __forceinline__ float dense_neg1()
+{
+    float v;
+    asm volatile("s_mov_b32 %0, -1.0" : "=s"(v));
+    return v;
 }
 
 // C += A_hi B_hi + A_lo B_hi + A_hi B_lo (f16 MFMA, FP32 accumulation; the
@@ -353,6 +363,7 @@ __global__ __launch_bounds__(64 * kDenseWaves) void m2l_dense_kernel(DenseArgs a
     }
     sexp = __builtin_amdgcn_readfirstlane(sexp);
     const float sc = ldexpf(1.0f, sexp);
+    const float neg1 = dense_neg1();
     // a wave-uniform float held in a scalar register
     auto uniform = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
 
@@ -445,10 +456,10 @@ __global__ __launch_bounds__(64 * kDenseWaves) void m2l_dense_kernel(DenseArgs a
                         }
                         const float xrj = xr[j], xij = xi[j];
                         _Float16 hh, ll;
-                        dense_split(xrj * cm - xij * sm, hh, ll);
+                        dense_split(xrj * cm - xij * sm, hh, ll, neg1);
                         bch[j] = hh;
                         bcl[j] = ll;
-                        dense_split(xrj * sm + xij * cm, hh, ll);
+                        dense_split(xrj * sm + xij * cm, hh, ll, neg1);
                         bsh[j] = hh;
                         bsl[j] = ll;
                     });
