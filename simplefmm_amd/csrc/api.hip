@@ -147,6 +147,7 @@ struct sfmm_plan_s {
     bool timed = false;
     int m2l_kernel = 0;
     sfmm::P2PItems p2p;                // near-field work items
+    int p2p_tpl = 2;                   // near-field tile shape: targets per lane (1, 2, 4)
     double* p2p_partial = nullptr;     // p2p.n_partial_rows x 4 doubles
     sfmm::M2LArgs m2l[5];              // offset-grouped M2L work per component count (built on first use)
     const double* rotA64 = nullptr;
@@ -216,9 +217,18 @@ struct sfmm_batch_s {
     bool own = false;               // every item on its plan's own stream
     std::vector<hipEvent_t> pjoin;      // per item, for `own`
     std::vector<hipEvent_t> pre, post;  // per item: its plan's stream before / after the item
+    // the launch sequence captured once: a HIP graph replayed by every launch
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
 
     ~sfmm_batch_s()
     {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (done) (void)hipEventDestroy(done);
+        if (cap) (void)hipStreamDestroy(cap);
         for (auto e : fork) (void)hipEventDestroy(e);
         for (auto e : join) (void)hipEventDestroy(e);
         for (auto e : pjoin) (void)hipEventDestroy(e);
@@ -622,7 +632,8 @@ int run_on(sfmm_plan_s* P, sfmm::Mode mode, int nc, int qdim, const double* dQ, 
     if (far) sfmm::launch_l2p(d, nc, P->L, P->obuf, s);
     REC(6);
     if (which & SFMM_NEAR) {
-        sfmm::launch_p2p(d, P->p2p, mode, nc, P->qbuf, P->obuf, P->p2p_partial, s, P->precision == SFMM_MIXED);
+        sfmm::launch_p2p(d, P->p2p, mode, nc, P->qbuf, P->obuf, P->p2p_partial, s, P->precision == SFMM_MIXED,
+                         P->p2p_tpl);
         sfmm::launch_self_correction(d, P->self, mode, nc, (which & SFMM_SELF_LIST) != 0, P->qbuf, P->obuf, s);
     }
     REC(7);
@@ -661,8 +672,11 @@ std::vector<int> batch_assign(const std::vector<sfmm_batch_item>& items, int ns)
 
 // Issues the items' evaluations on ns streams (own: each on its plan's stream)
 // after the work already queued on `origin`, and makes `origin` wait for all of them.
+// plan_order: each item also runs after the work queued on its plan's own stream,
+// and that stream's later work after the item (off while capturing: the plans'
+// streams are not part of the capture, and a captured launch orders them itself).
 int batch_issue(sfmm_batch_s* B, const std::vector<sfmm_batch_item>& items, const std::vector<int>& where, int ns,
-                hipStream_t origin, bool own = false)
+                hipStream_t origin, bool own = false, bool plan_order = true)
 {
     HIP_TRY(hipEventRecord(B->fork[0], origin));
     if (own) {
@@ -687,16 +701,67 @@ int batch_issue(sfmm_batch_s* B, const std::vector<sfmm_batch_item>& items, cons
         // the plan's buffers are shared with its own stream: the item runs after the
         // work already queued there, and later work there runs after the item
         hipStream_t bs = B->streams[where[i]];
-        HIP_TRY(hipEventRecord(B->pre[i], it.plan->stream));
-        HIP_TRY(hipStreamWaitEvent(bs, B->pre[i], 0));
+        if (plan_order) {
+            HIP_TRY(hipEventRecord(B->pre[i], it.plan->stream));
+            HIP_TRY(hipStreamWaitEvent(bs, B->pre[i], 0));
+        }
         if (int r = run_on(it.plan, mode, nc, qdim, it.dQ, it.dU, it.which, bs, false)) return r;
-        HIP_TRY(hipEventRecord(B->post[i], bs));
-        HIP_TRY(hipStreamWaitEvent(it.plan->stream, B->post[i], 0));
+        if (plan_order) {
+            HIP_TRY(hipEventRecord(B->post[i], bs));
+            HIP_TRY(hipStreamWaitEvent(it.plan->stream, B->post[i], 0));
+        }
     }
     for (int k = 0; k < ns; ++k) {
         HIP_TRY(hipEventRecord(B->join[k], B->streams[k]));
         HIP_TRY(hipStreamWaitEvent(origin, B->join[k], 0));
     }
+    return SFMM_OK;
+}
+
+// Captures the batch's launch sequence (its stream assignment) into a HIP graph
+// and instantiates it.  Inside the capture there are only kernel launches and
+// event records and waits between the capture stream and the worker streams it
+// forks (or, for `own`, the plans' streams, which join through the fork): every
+// buffer and work list exists beforehand, the fill kernel clears memory, and no
+// event recorded outside the capture is waited for.  The capture is always
+// ended, also after a failure; every status is checked.
+int batch_capture(sfmm_batch_s* B, const std::vector<sfmm_batch_item>& items, const std::vector<int>& where,
+                  int ns, bool own, hipGraph_t* graph, hipGraphExec_t* exec)
+{
+    *graph = nullptr;
+    *exec = nullptr;
+    HIP_TRY(hipStreamBeginCapture(B->cap, hipStreamCaptureModeRelaxed));
+    const int rc = batch_issue(B, items, where, ns, B->cap, own, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(B->cap, &g);
+    if (rc != SFMM_OK || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc != SFMM_OK) return rc;
+        return fail(SFMM_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e));
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e2 != hipSuccess || !x) {
+        (void)hipGraphDestroy(g);
+        return fail(SFMM_ERR_HIP, std::string("batch graph instantiate: ") + hipGetErrorString(e2));
+    }
+    *graph = g;
+    *exec = x;
+    return SFMM_OK;
+}
+
+// Replays a captured batch on `origin`: after the work queued on `origin` and on
+// every plan's own stream, and before the later work on both.
+int batch_replay(sfmm_batch_s* B, hipGraphExec_t exec, const std::vector<sfmm_batch_item>& items,
+                 hipStream_t origin)
+{
+    for (size_t i = 0; i < items.size(); ++i) {
+        HIP_TRY(hipEventRecord(B->pre[i], items[i].plan->stream));
+        HIP_TRY(hipStreamWaitEvent(origin, B->pre[i], 0));
+    }
+    HIP_TRY(hipGraphLaunch(exec, origin));
+    HIP_TRY(hipEventRecord(B->done, origin));
+    for (const auto& it : items) HIP_TRY(hipStreamWaitEvent(it.plan->stream, B->done, 0));
     return SFMM_OK;
 }
 
@@ -1426,9 +1491,13 @@ int sfmm_batch_create(const sfmm_batch_item* items, int count, int streams, sfmm
     for (auto& e : B->pre) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& e : B->post) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
+    HIP_TRY(hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&B->done, hipEventDisableTiming));
+
     int ns = std::min(streams, count);
     if (streams == 0) {
-        // tuning: time evaluations with 1, 2, 4, 8 streams into scratch outputs
+        // tuning, into scratch outputs: the near-field tile shape of each expansion
+        // order, then the stream assignment, each candidate timed as captured graphs
         std::vector<double*> scratch(count, nullptr);
         auto release = [&]() {
             for (double* p : scratch) (void)hipFree(p);
@@ -1445,30 +1514,71 @@ int sfmm_batch_create(const sfmm_batch_item* items, int count, int streams, sfmm
         HIP_TRY(hipEventCreate(&t0));
         HIP_TRY(hipEventCreate(&t1));
         hipStream_t s0 = B->streams[0];
-        float best = -1.0f;
         int rc = SFMM_OK;
-        // each candidate: one warm-up launch, then the best of three timed launches;
-        // more streams are taken only when at least 2% faster (streams beyond the
-        // hardware queues share them)
-        // the last candidate issues every item on its plan's own stream
+        // best of three timed runs after a warm-up
+        auto time_best = [&](const std::function<int()>& issue, float* best_ms) -> int {
+            *best_ms = -1.0f;
+            if (int r = issue()) return r;
+            for (int r = 0; r < 3; ++r) {
+                float ms = 0.0f;
+                HIP_TRY(hipEventRecord(t0, s0));
+                if (int r2 = issue()) return r2;
+                HIP_TRY(hipEventRecord(t1, s0));
+                HIP_TRY(hipEventSynchronize(t1));
+                HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+                if (*best_ms < 0.0f || ms < *best_ms) *best_ms = ms;
+            }
+            return SFMM_OK;
+        };
+        // near-field tile: 1, 2 or 4 targets per lane, timed on the largest problem of
+        // each order (its near field alone) and set on every plan of that order; a
+        // shape other than 2 is taken only when at least 2% faster
+        std::vector<int> orders;
+        for (const auto& it : B->items)
+            if (std::find(orders.begin(), orders.end(), it.plan->p) == orders.end()) orders.push_back(it.plan->p);
+        for (int p : orders) {
+            int big = -1;
+            for (int i = 0; i < count; ++i)
+                if (B->items[i].plan->p == p && (big < 0 || B->items[i].plan->tree->t.n > B->items[big].plan->tree->t.n))
+                    big = i;
+            sfmm_plan_s* P = B->items[big].plan;
+            sfmm::Mode mode;
+            int qdim, nc;
+            if ((rc = mode_of(trial[big].kind, trial[big].ncharge, mode, qdim, nc)) != SFMM_OK) break;
+            int best_tpl = 2;
+            float best_ms = -1.0f;
+            for (int tpl : {2, 1, 4}) {
+                P->p2p_tpl = tpl;
+                float ms = 0.0f;
+                rc = time_best([&]() { return run_on(P, mode, nc, qdim, trial[big].dQ, trial[big].dU, SFMM_NEAR, s0, false); },
+                               &ms);
+                if (rc != SFMM_OK) break;
+                if (best_ms < 0.0f || ms < 0.98f * best_ms) {
+                    best_ms = ms;
+                    best_tpl = tpl;
+                }
+            }
+            if (rc != SFMM_OK) break;
+            for (auto& it : B->items)
+                if (it.plan->p == p) it.plan->p2p_tpl = best_tpl;
+        }
+        // stream assignment: 1, 2, 4, 8 streams, then every item on its plan's own
+        // stream; more streams are taken only when at least 2% faster (streams beyond
+        // the hardware queues share them)
+        float best = -1.0f;
         for (int cand = 1; cand <= 2 * std::min(count, kBatchMaxStreams) && rc == SFMM_OK; cand *= 2) {
             const bool own = cand > std::min(count, kBatchMaxStreams);
             const std::vector<int> where = batch_assign(trial, own ? 1 : cand);
-            if ((rc = batch_issue(B.get(), trial, where, cand, s0, own)) != SFMM_OK) break;  // warm-up
+            hipGraph_t g = nullptr;
+            hipGraphExec_t x = nullptr;
+            if ((rc = batch_capture(B.get(), trial, where, cand, own, &g, &x)) != SFMM_OK) break;
             float cbest = -1.0f;
-            hipError_t e = hipSuccess;
-            for (int r = 0; r < 3 && e == hipSuccess && rc == SFMM_OK; ++r) {
-                float ms = 0.0f;
-                e = hipEventRecord(t0, s0);
-                if (e == hipSuccess) rc = batch_issue(B.get(), trial, where, cand, s0, own);
-                if (e == hipSuccess) e = hipEventRecord(t1, s0);
-                if (e == hipSuccess) e = hipEventSynchronize(t1);
-                if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
-                if (e == hipSuccess && (cbest < 0.0f || ms < cbest)) cbest = ms;
-            }
+            rc = time_best([&]() { return batch_replay(B.get(), x, trial, s0); }, &cbest);
+            (void)hipStreamSynchronize(s0);
+            (void)hipGraphExecDestroy(x);
+            (void)hipGraphDestroy(g);
             if (rc != SFMM_OK) break;
-            if (e != hipSuccess) rc = fail(SFMM_ERR_HIP, std::string("batch tuning: ") + hipGetErrorString(e));
-            else if (best < 0.0f || cbest < 0.98f * best) {
+            if (best < 0.0f || cbest < 0.98f * best) {
                 best = cbest;
                 ns = own ? 1 : cand;
                 B->own = own;
@@ -1482,6 +1592,9 @@ int sfmm_batch_create(const sfmm_batch_item* items, int count, int streams, sfmm
     }
     B->nstream = ns;
     B->where = batch_assign(B->items, ns);
+    if (int rc = batch_capture(B.get(), B->items, B->where, B->own ? static_cast<int>(B->items.size()) : ns, B->own,
+                               &B->graph, &B->exec))
+        return rc;
     *out = B.release();
     return SFMM_OK;
 }
@@ -1490,8 +1603,7 @@ int sfmm_batch_launch(sfmm_batch batch, void* stream)
 {
     if (!batch) return fail(SFMM_ERR_ARG, "null argument");
     HIP_TRY(hipSetDevice(batch->device));
-    return batch_issue(batch, batch->items, batch->where, batch->nstream,
-                       stream ? static_cast<hipStream_t>(stream) : batch->streams[0], batch->own);
+    return batch_replay(batch, batch->exec, batch->items, stream ? static_cast<hipStream_t>(stream) : batch->streams[0]);
 }
 
 int sfmm_batch_synchronize(sfmm_batch batch)

@@ -564,19 +564,20 @@ __device__ __forceinline__ void pair_kernel_f32(float dx, float dy, float dz, co
 // a target leaf and a run of at most a few thousand sources of its near list
 // (the concatenation of its merged source ranges): leaves whose near list is
 // long (coarse leaves next to deep subtrees) are split over several items so
-// that no wavefront carries a long tail.  Each lane holds two targets (so every
-// source read from LDS serves two pairs, and the two pair computations are
-// independent); the U = ceil(nt / 2) target pairs of a leaf are spread over
+// that no wavefront carries a long tail.  Each lane holds T targets (T = 1, 2
+// or 4, the tile shape: every source read from LDS serves T independent
+// pairs); the U = ceil(nt / T) target sets of a leaf are spread over
 // S = floor(64 / U) lanes each, which take every S-th source.  Sources are
-// staged through LDS 64 at a time; only tiles that hold particles of the
-// target leaf itself test for the pair (i, i) (the reference's default
-// is_self, fmm.hpp:875-881).  An unsplit item adds into the output; the items
-// of a split leaf write partial sums that p2p_reduce_kernel adds in a fixed
-// order, so results are bitwise repeatable.
+// staged through LDS 64 at a time, fetched one tile ahead; only tiles that
+// hold particles of the target leaf itself test for the pair (i, i) (the
+// reference's default is_self, fmm.hpp:875-881).  An unsplit item adds into
+// the output; the items of a split leaf write partial sums that
+// p2p_reduce_kernel adds in a fixed order, so results are bitwise repeatable
+// for a given T.
 // MIXED: coordinate differences in FP64 rounded to FP32 (so close pairs keep
 // their relative precision), FP32 pair arithmetic, FP32 sums per source tile
 // added into FP64 accumulators.
-template <int MODE, int NCQ, int NOUT, bool MIXED>
+template <int MODE, int NCQ, int NOUT, bool MIXED, int T>
 __global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, P2PItems w, const double* __restrict__ q,
                                                  double* __restrict__ out, double* __restrict__ partial)
 {
@@ -584,7 +585,7 @@ __global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, P2PItems w, const do
     __shared__ double sx[64], sy[64], sz[64];
     __shared__ R sq[NCQ][64];
     __shared__ int32_t sidx[64];
-    __shared__ double red[2][NOUT][64];
+    __shared__ double red[T][NOUT][64];
     const int64_t it = blockIdx.x;
     const int tid = threadIdx.x;
     const int64_t g = w.leaf[it];
@@ -593,33 +594,37 @@ __global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, P2PItems w, const do
     const int64_t tb = d.lf_begin[g], te = d.lf_end[g];
     const int64_t nt = te - tb;
     const int64_t r0 = d.near_row[g], r1 = d.near_row[g + 1];
-    const int ntp = nt < 128 ? static_cast<int>(nt) : 128;  // targets per pass
-    const int U = (ntp + 1) / 2;                              // lanes' target pairs per pass
+    const int ntp = nt < 64 * T ? static_cast<int>(nt) : 64 * T;  // targets per pass
+    const int U = (ntp + T - 1) / T;                              // lanes' target sets per pass
     const int S = 64 / U;
     const int t = tid % U, s = tid / U;
     const bool lane_on = s < S;
     for (int64_t t0 = tb; t0 < te; t0 += ntp) {
-        // this lane's targets: t0 + t and t0 + U + t
-        const int64_t slot_a = t0 + t, slot_b = t0 + U + t;
-        const bool va = lane_on && slot_a < te, vb = lane_on && slot_b < te;
-        double xa = 0, ya = 0, za = 0, xb = 0, yb = 0, zb = 0;
-        if (va) {
-            xa = d.px[slot_a];
-            ya = d.py[slot_a];
-            za = d.pz[slot_a];
-        }
-        if (vb) {
-            xb = d.px[slot_b];
-            yb = d.py[slot_b];
-            zb = d.pz[slot_b];
-        }
-        // the targets' positions are waited for here, not inside the pair loop, where
-        // in-order load counting would also wait for the next tile's prefetch
-        asm volatile("" ::"v"(xa), "v"(ya), "v"(za), "v"(xb), "v"(yb), "v"(zb));
-        const int32_t ia = static_cast<int32_t>(slot_a), ib = static_cast<int32_t>(slot_b);
-        double acc_a[NOUT], acc_b[NOUT];
+        // this lane's targets: t0 + t + U h, h < T
+        int64_t slot[T];
+        bool val[T];
+        double xt[T], yt[T], zt[T];
+        int32_t it32[T];
 #pragma unroll
-        for (int c = 0; c < NOUT; ++c) acc_a[c] = acc_b[c] = 0.0;
+        for (int h = 0; h < T; ++h) {
+            slot[h] = t0 + t + static_cast<int64_t>(U) * h;
+            val[h] = lane_on && slot[h] < te;
+            xt[h] = yt[h] = zt[h] = 0.0;
+            if (val[h]) {
+                xt[h] = d.px[slot[h]];
+                yt[h] = d.py[slot[h]];
+                zt[h] = d.pz[slot[h]];
+            }
+            // waited for here, not inside the pair loop, where in-order load
+            // counting would also wait for the next tile's prefetch
+            asm volatile("" ::"v"(xt[h]), "v"(yt[h]), "v"(zt[h]));
+            it32[h] = static_cast<int32_t>(slot[h]);
+        }
+        double acc[T][NOUT];
+#pragma unroll
+        for (int h = 0; h < T; ++h)
+#pragma unroll
+            for (int c = 0; c < NOUT; ++c) acc[h][c] = 0.0;
         int64_t rc = r0;  // range holding source u of this lane, advanced monotonically
         // this lane's source of a tile, loaded one tile ahead of its use
         double fx = 0, fy = 0, fz = 0;
@@ -654,70 +659,66 @@ __global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, P2PItems w, const do
             __syncthreads();
             if (ut + 64 < u1) fetch(ut + 64);
             const int cnt = static_cast<int>(u1 - ut < 64 ? u1 - ut : 64);
-            if (va) {
-                R a[NOUT], b[NOUT];
+            if (val[0]) {
+                R a[T][NOUT];
 #pragma unroll
-                for (int c = 0; c < NOUT; ++c) a[c] = b[c] = 0;
+                for (int h = 0; h < T; ++h)
+#pragma unroll
+                    for (int c = 0; c < NOUT; ++c) a[h][c] = 0;
                 auto pairs = [&](auto checked) {
                     for (int k = s; k < cnt; k += S) {
-                        R qa[NCQ], qb[NCQ];
+                        R qs[NCQ];
 #pragma unroll
-                        for (int c = 0; c < NCQ; ++c) qa[c] = qb[c] = sq[c][k];
+                        for (int c = 0; c < NCQ; ++c) qs[c] = sq[c][k];
                         const double xj = sx[k], yj = sy[k], zj = sz[k];
-                        double dxa = xa - xj, dya = ya - yj, dza = za - zj;
-                        double dxb = xb - xj, dyb = yb - yj, dzb = zb - zj;
-                        if constexpr (decltype(checked)::value) {
-                            // the pair (i, i) contributes nothing: unit distance, zero charge
-                            const int32_t sj = sidx[k];
-                            if (sj == ia) {
-                                dxa = 1.0;
+                        const int32_t sj = decltype(checked)::value ? sidx[k] : -1;
 #pragma unroll
-                                for (int c = 0; c < NCQ; ++c) qa[c] = 0;
-                            }
-                            if (sj == ib) {
-                                dxb = 1.0;
+                        for (int h = 0; h < T; ++h) {
+                            R qh[NCQ];
 #pragma unroll
-                                for (int c = 0; c < NCQ; ++c) qb[c] = 0;
+                            for (int c = 0; c < NCQ; ++c) qh[c] = qs[c];
+                            double dx = xt[h] - xj;
+                            const double dy = yt[h] - yj, dz = zt[h] - zj;
+                            if constexpr (decltype(checked)::value) {
+                                // the pair (i, i) contributes nothing: unit distance, zero charge
+                                if (sj == it32[h]) {
+                                    dx = 1.0;
+#pragma unroll
+                                    for (int c = 0; c < NCQ; ++c) qh[c] = 0;
+                                }
                             }
-                        }
-                        if constexpr (MIXED) {
-                            pair_kernel_f32<MODE, NCQ>(static_cast<float>(dxa), static_cast<float>(dya),
-                                                       static_cast<float>(dza), qa, a);
-                            pair_kernel_f32<MODE, NCQ>(static_cast<float>(dxb), static_cast<float>(dyb),
-                                                       static_cast<float>(dzb), qb, b);
-                        } else {
-                            pair_kernel_nr<MODE, NCQ>(dxa, dya, dza, qa, a);
-                            pair_kernel_nr<MODE, NCQ>(dxb, dyb, dzb, qb, b);
+                            if constexpr (MIXED)
+                                pair_kernel_f32<MODE, NCQ>(static_cast<float>(dx), static_cast<float>(dy),
+                                                           static_cast<float>(dz), qh, a[h]);
+                            else
+                                pair_kernel_nr<MODE, NCQ>(dx, dy, dz, qh, a[h]);
                         }
                     }
                 };
                 if (own) pairs(std::true_type{});
                 else pairs(std::false_type{});
 #pragma unroll
-                for (int c = 0; c < NOUT; ++c) {
-                    acc_a[c] += static_cast<double>(a[c]);
-                    acc_b[c] += static_cast<double>(b[c]);
-                }
+                for (int h = 0; h < T; ++h)
+#pragma unroll
+                    for (int c = 0; c < NOUT; ++c) acc[h][c] += static_cast<double>(a[h][c]);
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int c = 0; c < NOUT; ++c) {
-            red[0][c][tid] = acc_a[c];
-            red[1][c][tid] = acc_b[c];
-        }
+        for (int h = 0; h < T; ++h)
+#pragma unroll
+            for (int c = 0; c < NOUT; ++c) red[h][c][tid] = acc[h][c];
         __syncthreads();
         if (s == 0) {
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int64_t slot = h == 0 ? slot_a : slot_b;
-                if (slot >= te) continue;
+            for (int h = 0; h < T; ++h) {
+                if (slot[h] >= te) continue;
 #pragma unroll
                 for (int c = 0; c < NOUT; ++c) {
                     double v = 0.0;
                     for (int ss = 0; ss < S; ++ss) v += red[h][c][t + ss * U];
-                    if (pslot < 0) out[c * d.n + slot] += v;
-                    else partial[(pslot + (slot - tb)) * NOUT + c] = v;
+                    if (pslot < 0) out[c * d.n + slot[h]] += v;
+                    else partial[(pslot + (slot[h] - tb)) * NOUT + c] = v;
                 }
             }
         }
@@ -941,24 +942,24 @@ void launch_l2l(const DevPlan& d, LevelSpan children, int nc, double* L, hipStre
                        sizeof(double) * nc * d.K, s, d, children.first, nc, L);
 }
 
-template <bool MIXED>
+template <bool MIXED, int T>
 void launch_p2p_t(const DevPlan& d, const P2PItems& w, Mode mode, int nc, const double* q, double* out,
                   double* partial, hipStream_t s)
 {
     const dim3 grid(static_cast<unsigned>(w.n_item)), blk(64);
     int nout = nc;
     if (mode == MODE_POTGRAD) {
-        hipLaunchKernelGGL((p2p_kernel<MODE_POTGRAD, 1, 4, MIXED>), grid, blk, 0, s, d, w, q, out, partial);
+        hipLaunchKernelGGL((p2p_kernel<MODE_POTGRAD, 1, 4, MIXED, T>), grid, blk, 0, s, d, w, q, out, partial);
         nout = 4;
     } else if (mode == MODE_CURL) {
-        hipLaunchKernelGGL((p2p_kernel<MODE_CURL, 3, 3, MIXED>), grid, blk, 0, s, d, w, q, out, partial);
+        hipLaunchKernelGGL((p2p_kernel<MODE_CURL, 3, 3, MIXED, T>), grid, blk, 0, s, d, w, q, out, partial);
         nout = 3;
     } else {
         switch (nc) {
-        case 1: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 1, 1, MIXED>), grid, blk, 0, s, d, w, q, out, partial); break;
-        case 2: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 2, 2, MIXED>), grid, blk, 0, s, d, w, q, out, partial); break;
-        case 3: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 3, 3, MIXED>), grid, blk, 0, s, d, w, q, out, partial); break;
-        default: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 4, 4, MIXED>), grid, blk, 0, s, d, w, q, out, partial); break;
+        case 1: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 1, 1, MIXED, T>), grid, blk, 0, s, d, w, q, out, partial); break;
+        case 2: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 2, 2, MIXED, T>), grid, blk, 0, s, d, w, q, out, partial); break;
+        case 3: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 3, 3, MIXED, T>), grid, blk, 0, s, d, w, q, out, partial); break;
+        default: hipLaunchKernelGGL((p2p_kernel<MODE_POT, 4, 4, MIXED, T>), grid, blk, 0, s, d, w, q, out, partial); break;
         }
     }
     if (w.n_split_targets > 0)
@@ -967,11 +968,17 @@ void launch_p2p_t(const DevPlan& d, const P2PItems& w, Mode mode, int nc, const 
 }
 
 void launch_p2p(const DevPlan& d, const P2PItems& w, Mode mode, int nc, const double* q, double* out,
-                double* partial, hipStream_t s, bool mixed)
+                double* partial, hipStream_t s, bool mixed, int tpl)
 {
     if (w.n_item == 0) return;
-    if (mixed) launch_p2p_t<true>(d, w, mode, nc, q, out, partial, s);
-    else launch_p2p_t<false>(d, w, mode, nc, q, out, partial, s);
+    switch (tpl) {
+    case 1: mixed ? launch_p2p_t<true, 1>(d, w, mode, nc, q, out, partial, s)
+                  : launch_p2p_t<false, 1>(d, w, mode, nc, q, out, partial, s); break;
+    case 4: mixed ? launch_p2p_t<true, 4>(d, w, mode, nc, q, out, partial, s)
+                  : launch_p2p_t<false, 4>(d, w, mode, nc, q, out, partial, s); break;
+    default: mixed ? launch_p2p_t<true, 2>(d, w, mode, nc, q, out, partial, s)
+                   : launch_p2p_t<false, 2>(d, w, mode, nc, q, out, partial, s); break;
+    }
 }
 
 void launch_self_correction(const DevPlan& d, const SelfCorr& c, Mode mode, int nc, bool use_near, const double* q,

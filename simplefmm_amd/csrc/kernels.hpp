@@ -151,8 +151,9 @@ int translate_nfrag(int p, bool upper);
 int translate_frag_index(int p, bool upper, int row, int col, int* lane);
 // mixed: FP32 pair arithmetic with FP64 accumulation per source tile.
 // partial: P2PItems::n_partial_rows x nout doubles of scratch.
+// tpl: targets per lane (1, 2 or 4), the kernel's tile shape.
 void launch_p2p(const DevPlan& d, const P2PItems& w, Mode mode, int nc, const double* q, double* out,
-                double* partial, hipStream_t s, bool mixed);
+                double* partial, hipStream_t s, bool mixed, int tpl = 2);
 // Subtracts the pairs of the self lists: far ones always, near ones when
 // use_near (the near field then used the lists as its exclusion predicate).
 void launch_self_correction(const DevPlan& d, const SelfCorr& c, Mode mode, int nc, bool use_near, const double* q,
