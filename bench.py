@@ -275,6 +275,10 @@ def main() -> int:
         t_a = time.time()
         fmm = api.FMM3D(part, p, ncharge=1, precision=api.MIXED if prec == "mixed" else api.FP64)
         t_b = time.time()
+        # the kernels' tile shapes, timed on this plan (outside the timed steps)
+        fmm.tune(api.POTGRAD, 1, dq.data_ptr())
+        tiles = fmm.tiles()
+        log(f"[bench] {prec}: tiles {tiles} (tuned in {time.time() - t_b:.2f}s)")
 
         def step():
             fmm.evaluate_device(api.POTGRAD, 1, dq.data_ptr(), du.data_ptr())
@@ -323,7 +327,7 @@ def main() -> int:
         achieved = flops_complex / (m2l_ms * 1e-3)
         del fmm
         return {
-            "points_per_sec": n / (ms * 1e-3), "ms_per_step": ms, "plan_build_s": t_b - t_a,
+            "points_per_sec": n / (ms * 1e-3), "ms_per_step": ms, "plan_build_s": t_b - t_a, "tiles": tiles,
             "error": {"targets": int(len(tg)), "rel_l2_phi": err_phi, "rel_l2_grad": err_grad,
                       "within_1e-6": bool(max(err_phi, err_grad) <= 1e-6)},
             "stages_ms": mean,

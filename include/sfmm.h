@@ -134,6 +134,16 @@ int sfmm_plan_slist(sfmm_plan plan, int64_t* row, int64_t* val, int64_t* nnz);
 
 /* The plan's HIP stream (hipStream_t) for sfmm_evaluate_device. */
 int sfmm_plan_stream(sfmm_plan plan, void** stream);
+
+/* No counterpart in the reference (its kernels have no tile shapes).  Times the
+ * kernels' tile shapes on this plan with one evaluation's device charges (as for
+ * sfmm_evaluate_device): the near field's targets per lane (1, 2, 4) and, for the
+ * mixed path at p <= 12, the M2L's wavefronts per workgroup (4, 8); keeps the
+ * fastest for later evaluations.  A shape changes the rounding, not the method.
+ * Synchronous; allocates scratch outputs while it runs. */
+int sfmm_plan_tune(sfmm_plan plan, int kind, int ncharge, const double* dQ);
+/* The plan's tile shapes (m2l_waves 0: the plan does not use the dense M2L). */
+int sfmm_plan_tiles(sfmm_plan plan, int* p2p_targets_per_lane, int* m2l_waves);
 int sfmm_synchronize(sfmm_plan plan);
 
 /* Device milliseconds of each stage of the last evaluation:
@@ -181,13 +191,18 @@ typedef struct {
     double* dU;         /* device outputs, accumulated */
 } sfmm_batch_item;
 
-/* Prepares every plan for its item and assigns the items to concurrent streams
- * (largest first).  streams: 1..8, or 0 to time evaluations with 1, 2, 4, 8
- * streams and with every item on its plan's own stream (into scratch outputs)
- * and keep the fastest.  The items' arrays must stay valid while the batch is used. */
+/* Prepares every plan for its item, assigns the items to concurrent streams
+ * (largest first) and captures the launch sequence into a HIP graph that every
+ * launch replays.  streams: 1..8, or 0 to tune first (into scratch outputs): the
+ * tile shapes of each expansion order on its largest problem (as
+ * sfmm_plan_tune; set on every plan of that order), then 1, 2, 4, 8 streams and
+ * every item on its plan's own stream, each timed as graph replays, keeping the
+ * fastest.  The items' arrays must stay valid while the batch is used; a batch
+ * keeps the tile shapes its plans had when it was created. */
 int sfmm_batch_create(const sfmm_batch_item* items, int count, int streams, sfmm_batch* out);
-/* Evaluates every item after the work queued on `stream` (hipStream_t; NULL: the
- * batch's own stream), which then waits for all of them; asynchronous. */
+/* Evaluates every item (one graph launch) after the work queued on `stream`
+ * (hipStream_t; NULL: the batch's own stream) and on each plan's own stream, both
+ * of which then wait for all of them; asynchronous. */
 int sfmm_batch_launch(sfmm_batch batch, void* stream);
 /* Waits for launches on the batch's own stream. */
 int sfmm_batch_synchronize(sfmm_batch batch);

@@ -80,6 +80,8 @@ SIGNATURES = {
     "sfmm_evaluate_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_int]),
     "sfmm_plan_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "sfmm_plan_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "sfmm_plan_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "sfmm_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "sfmm_last_stage_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "sfmm_record_stages": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -306,6 +308,16 @@ class FMM3D:
 
     def synchronize(self):
         _check(lib().sfmm_synchronize(self.h))
+
+    def tune(self, kind: int, ncharge: int, q_ptr: int):
+        """Times the kernels' tile shapes with these device charges and keeps the fastest."""
+        _check(lib().sfmm_plan_tune(self.h, kind, ncharge, ctypes.c_void_p(q_ptr)))
+
+    def tiles(self) -> dict:
+        """{'p2p_targets_per_lane': T, 'm2l_waves': W} (W = 0 without the dense M2L)."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(lib().sfmm_plan_tiles(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return {"p2p_targets_per_lane": a.value, "m2l_waves": b.value}
 
     def slist(self):
         """The far self pairs (_slist, fmm.hpp:398-408): (row, val) CSR over original indices."""

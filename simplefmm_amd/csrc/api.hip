@@ -167,6 +167,7 @@ struct sfmm_plan_s {
     float* dense_nodemax = nullptr;     // largest |value| per node and component of the FP32 multipoles
     int dense_op_exp = 0;
     int dense_kd = 0;                   // floats per (node, component) of the FP32 copy
+    int dense_waves = 8;                // the dense M2L's tile shape: wavefronts per workgroup (4, 8)
     std::vector<int32_t> dense_grp;     // azimuth group per offset id
     sfmm::SelfCorr self;                // self-list corrections (empty without a self generator)
     std::vector<int64_t> slist_row, slist_val;  // far self pairs per particle, original order (_slist)
@@ -391,7 +392,7 @@ int build_dense_blocks(sfmm_plan_s* P, int nc)
     if (P->dns[nc].blk_row) return SFMM_OK;
     // targets per block: 16 per group of waves, one wave per component
     const int ncw = nc == 3 ? 4 : nc;
-    const int TB = 16 * (sfmm::kDenseWaves / ncw);
+    const int TB = 16 * (P->dense_waves / ncw);
     const auto& cls = P->tables.off_class;
     const auto& grp = P->dense_grp;
     std::vector<int> order;
@@ -416,8 +417,19 @@ int build_dense_blocks(sfmm_plan_s* P, int nc)
     A.nodemax = P->dense_nodemax;
     A.zero = P->dense_zero;
     A.op_exp = P->dense_op_exp;
+    A.waves = P->dense_waves;
     P->dns[nc] = A;
     return SFMM_OK;
+}
+
+// Sets the dense M2L's wavefronts per workgroup; its work lists (cut for the
+// previous shape) are rebuilt on next use.  The previous lists stay allocated
+// until the plan is destroyed: a batch created earlier may have captured them.
+void set_dense_waves(sfmm_plan_s* P, int waves)
+{
+    if (P->dense_waves == waves) return;
+    for (auto& A : P->dns) A = sfmm::DenseArgs{};
+    P->dense_waves = waves;
 }
 
 // Groups the targets of each level by child octant into blocks of 16/nc and
@@ -748,6 +760,80 @@ int batch_capture(sfmm_batch_s* B, const std::vector<sfmm_batch_item>& items, co
     *graph = g;
     *exec = x;
     return SFMM_OK;
+}
+
+// Times the tile shapes of plan P for an evaluation `it` (its dU is scratch) on
+// stream s and keeps the fastest: the near field's targets per lane (1, 2, 4;
+// near field alone) when `near`, the dense M2L's wavefronts per workgroup (8, 4;
+// far field alone) when `far` and the plan uses it.  Each shape is timed as the
+// best of three runs after a warm-up; a shape other than the default (2 and 8)
+// is kept only when at least 2% faster.  The plan must be prepared for `it`.
+int tune_tiles(sfmm_plan_s* P, const sfmm_batch_item& it, hipStream_t s, bool near, bool far, int* tpl_out,
+               int* waves_out)
+{
+    sfmm::Mode mode;
+    int qdim, nc;
+    if (int rc = mode_of(it.kind, it.ncharge, mode, qdim, nc)) return rc;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    int rc = SFMM_OK;
+    auto timed = [&](int which, float* best) {
+        *best = -1.0f;
+        for (int r = 0; r < 4 && rc == SFMM_OK; ++r) {
+            float ms = 0.0f;
+            hipError_t e = hipEventRecord(t0, s);
+            if (e == hipSuccess) rc = run_on(P, mode, nc, qdim, it.dQ, it.dU, which, s, false);
+            if (rc != SFMM_OK) return;
+            if (e == hipSuccess) e = hipEventRecord(t1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(t1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+            if (e != hipSuccess) {
+                rc = fail(SFMM_ERR_HIP, std::string("tile tuning: ") + hipGetErrorString(e));
+                return;
+            }
+            if (r > 0 && (*best < 0.0f || ms < *best)) *best = ms;  // run 0 warms up
+        }
+    };
+    if (near) {
+        int best_tpl = 2;
+        float best_ms = -1.0f;
+        for (int tpl : {2, 1, 4}) {
+            P->p2p_tpl = tpl;
+            float ms = 0.0f;
+            timed(SFMM_NEAR, &ms);
+            if (rc != SFMM_OK) break;
+            if (best_ms < 0.0f || ms < 0.98f * best_ms) {
+                best_ms = ms;
+                best_tpl = tpl;
+            }
+        }
+        P->p2p_tpl = best_tpl;
+        if (tpl_out) *tpl_out = best_tpl;
+    }
+    if (far && P->dense && rc == SFMM_OK) {
+        int best_w = 8;
+        float best_ms = -1.0f;
+        for (int w : {8, 4}) {
+            set_dense_waves(P, w);
+            if ((rc = prepare(P, it.kind, it.ncharge, it.which, mode, qdim, nc)) != SFMM_OK) break;
+            float ms = 0.0f;
+            timed(SFMM_FAR, &ms);
+            if (rc != SFMM_OK) break;
+            if (best_ms < 0.0f || ms < 0.98f * best_ms) {
+                best_ms = ms;
+                best_w = w;
+            }
+        }
+        if (rc == SFMM_OK) {
+            set_dense_waves(P, best_w);
+            rc = prepare(P, it.kind, it.ncharge, it.which, mode, qdim, nc);
+        }
+        if (waves_out) *waves_out = best_w;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return rc;
 }
 
 // Replays a captured batch on `origin`: after the work queued on `origin` and on
@@ -1274,6 +1360,33 @@ int sfmm_evaluate_device(sfmm_plan plan, int kind, int ncharge, const double* dQ
     return run(plan, kind, ncharge, dQ, dU, which);
 }
 
+int sfmm_plan_tune(sfmm_plan plan, int kind, int ncharge, const double* dQ)
+{
+    if (!plan || !dQ) return fail(SFMM_ERR_ARG, "null argument");
+    sfmm::Mode mode;
+    int qdim, nc;
+    HIP_TRY(hipSetDevice(plan->device));
+    if (int rc = prepare(plan, kind, ncharge, SFMM_BOTH, mode, qdim, nc)) return rc;
+    double* scratch = nullptr;
+    if (dev_malloc(&scratch, sizeof(double) * nc * plan->tree->t.n) != hipSuccess)
+        return fail(SFMM_ERR_ALLOC, "hipMalloc failed (tuning outputs)");
+    const sfmm_batch_item it{plan, kind, ncharge, SFMM_BOTH, dQ, scratch};
+    int rc = tune_tiles(plan, it, plan->stream, true, true, nullptr, nullptr);
+    const hipError_t e = hipStreamSynchronize(plan->stream);
+    (void)hipFree(scratch);
+    if (rc != SFMM_OK) return rc;
+    if (e != hipSuccess) return fail(SFMM_ERR_HIP, std::string("tuning: ") + hipGetErrorString(e));
+    return SFMM_OK;
+}
+
+int sfmm_plan_tiles(sfmm_plan plan, int* p2p_targets_per_lane, int* m2l_waves)
+{
+    if (!plan || !p2p_targets_per_lane || !m2l_waves) return fail(SFMM_ERR_ARG, "null argument");
+    *p2p_targets_per_lane = plan->p2p_tpl;
+    *m2l_waves = plan->dense ? plan->dense_waves : 0;
+    return SFMM_OK;
+}
+
 int sfmm_evaluate(sfmm_plan plan, int kind, int ncharge, const double* Q, double* U, int which)
 {
     if (!plan || !Q || !U) return fail(SFMM_ERR_ARG, "null argument");
@@ -1530,37 +1643,34 @@ int sfmm_batch_create(const sfmm_batch_item* items, int count, int streams, sfmm
             }
             return SFMM_OK;
         };
-        // near-field tile: 1, 2 or 4 targets per lane, timed on the largest problem of
-        // each order (its near field alone) and set on every plan of that order; a
-        // shape other than 2 is taken only when at least 2% faster
+        // tile shapes (near field, dense M2L) of each expansion order, timed on its
+        // largest problem and set on every plan of that order
         std::vector<int> orders;
         for (const auto& it : B->items)
             if (std::find(orders.begin(), orders.end(), it.plan->p) == orders.end()) orders.push_back(it.plan->p);
         for (int p : orders) {
-            int big = -1;
-            for (int i = 0; i < count; ++i)
-                if (B->items[i].plan->p == p && (big < 0 || B->items[i].plan->tree->t.n > B->items[big].plan->tree->t.n))
-                    big = i;
-            sfmm_plan_s* P = B->items[big].plan;
-            sfmm::Mode mode;
-            int qdim, nc;
-            if ((rc = mode_of(trial[big].kind, trial[big].ncharge, mode, qdim, nc)) != SFMM_OK) break;
-            int best_tpl = 2;
-            float best_ms = -1.0f;
-            for (int tpl : {2, 1, 4}) {
-                P->p2p_tpl = tpl;
-                float ms = 0.0f;
-                rc = time_best([&]() { return run_on(P, mode, nc, qdim, trial[big].dQ, trial[big].dU, SFMM_NEAR, s0, false); },
-                               &ms);
-                if (rc != SFMM_OK) break;
-                if (best_ms < 0.0f || ms < 0.98f * best_ms) {
-                    best_ms = ms;
-                    best_tpl = tpl;
-                }
+            int big = -1, big_dense = -1;
+            for (int i = 0; i < count; ++i) {
+                const sfmm_plan_s* Q = B->items[i].plan;
+                if (Q->p != p) continue;
+                if (big < 0 || Q->tree->t.n > B->items[big].plan->tree->t.n) big = i;
+                if (Q->dense && (big_dense < 0 || Q->tree->t.n > B->items[big_dense].plan->tree->t.n)) big_dense = i;
             }
-            if (rc != SFMM_OK) break;
-            for (auto& it : B->items)
-                if (it.plan->p == p) it.plan->p2p_tpl = best_tpl;
+            int tpl = 2, waves = 8;
+            if ((rc = tune_tiles(B->items[big].plan, trial[big], s0, true, false, &tpl, nullptr)) != SFMM_OK) break;
+            if (big_dense >= 0 &&
+                (rc = tune_tiles(B->items[big_dense].plan, trial[big_dense], s0, false, true, nullptr, &waves)) != SFMM_OK)
+                break;
+            for (int i = 0; i < count && rc == SFMM_OK; ++i) {
+                sfmm_plan_s* Q = B->items[i].plan;
+                if (Q->p != p) continue;
+                Q->p2p_tpl = tpl;
+                if (!Q->dense) continue;
+                set_dense_waves(Q, waves);
+                sfmm::Mode mode;
+                int qdim, nc;
+                rc = prepare(Q, B->items[i].kind, B->items[i].ncharge, B->items[i].which, mode, qdim, nc);
+            }
         }
         // stream assignment: 1, 2, 4, 8 streams, then every item on its plan's own
         // stream; more streams are taken only when at least 2% faster (streams beyond

@@ -85,6 +85,7 @@ struct DenseArgs {
     const float* nodemax = nullptr;     // [gnode][nc] largest |value| of each FP32 multipole component
     const float* zero = nullptr;        // zeros: the columns of absent sources
     int op_exp = 0;                     // the operators are stored scaled by 2^op_exp
+    int waves = 8;                      // wavefronts per workgroup (4 or 8): 16 targets per wavefront group
 };
 
 // Near-field work: items of (leaf, run of its concatenated near sources).

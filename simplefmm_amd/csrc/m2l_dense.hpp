@@ -11,7 +11,7 @@
 // diagonal: a cos block on the Re parts (m >= 0) and a sin block on the Im parts
 // (m >= 1).  This kernel stores D_c densely (70 classes, level independent) and
 // applies it as one matrix product per interaction offset:
-//   * a workgroup (8 wavefronts, two per SIMD) takes a block of targets of one
+//   * a workgroup (8 wavefronts, two per SIMD, or 4) takes a block of targets of one
 //     level and child octant (they share their 189 candidate offsets); wave w
 //     holds component w % NCW of a group of 16 targets as the 16 columns of
 //     16x16x32 f16 MFMA tiles;
@@ -40,7 +40,7 @@
 namespace sfmm {
 
 constexpr int kDenseMaxP = 12;
-constexpr int kDenseWaves = 8;         // wavefronts per workgroup (two per SIMD)
+constexpr int kDenseWaves = 8;         // most wavefronts per workgroup (two per SIMD); 4 also runs
 constexpr int kDenseMaxEntries = 192;  // interaction offsets of a block (at most 189)
 constexpr int kDenseMaxSlots = 32;
 constexpr int kDenseMaxRows = 112;
@@ -304,13 +304,12 @@ __device__ __forceinline__ void dense_flush(df4* cc, df4* cs, df4* sumc, df4* su
     });
 }
 
-template <int P, int NC>
-__global__ __launch_bounds__(64 * kDenseWaves) void m2l_dense_kernel(DenseArgs a, const float* __restrict__ M32,
-                                                                    double* __restrict__ L)
+template <int P, int NC, int NW>
+__global__ __launch_bounds__(64 * NW) void m2l_dense_kernel(DenseArgs a, const float* __restrict__ M32,
+                                                           double* __restrict__ L)
 {
     using Cfg = DenseCfg<P>;
     constexpr int K = (P + 1) * (P + 1);
-    constexpr int NW = kDenseWaves;
     constexpr int NCW = NC == 3 ? 4 : NC;  // waves per target group (NC = 3: the fourth only copies)
     constexpr int G = NW / NCW;            // target groups per block
     constexpr int TB = 16 * G;             // targets per block
@@ -518,16 +517,25 @@ __global__ __launch_bounds__(64 * kDenseWaves) void m2l_dense_kernel(DenseArgs a
     });
 }
 
+template <int P, int NW>
+void launch_m2l_dense_pw(const DenseArgs& a, int nc, const float* M32, double* L, hipStream_t s)
+{
+    const dim3 grid(static_cast<unsigned>(8 * ((a.n_block + 7) / 8))), blk(64 * NW);
+    switch (nc) {
+    case 1: hipLaunchKernelGGL((m2l_dense_kernel<P, 1, NW>), grid, blk, 0, s, a, M32, L); break;
+    case 2: hipLaunchKernelGGL((m2l_dense_kernel<P, 2, NW>), grid, blk, 0, s, a, M32, L); break;
+    case 3: hipLaunchKernelGGL((m2l_dense_kernel<P, 3, NW>), grid, blk, 0, s, a, M32, L); break;
+    default: hipLaunchKernelGGL((m2l_dense_kernel<P, 4, NW>), grid, blk, 0, s, a, M32, L); break;
+    }
+}
+
+// a.waves (4 or 8) wavefronts per workgroup: the tile shape, which fixes the
+// targets per block the work lists were cut for (16 per wavefront group).
 template <int P>
 void launch_m2l_dense_p(const DenseArgs& a, int nc, const float* M32, double* L, hipStream_t s)
 {
-    const dim3 grid(static_cast<unsigned>(8 * ((a.n_block + 7) / 8))), blk(64 * kDenseWaves);
-    switch (nc) {
-    case 1: hipLaunchKernelGGL((m2l_dense_kernel<P, 1>), grid, blk, 0, s, a, M32, L); break;
-    case 2: hipLaunchKernelGGL((m2l_dense_kernel<P, 2>), grid, blk, 0, s, a, M32, L); break;
-    case 3: hipLaunchKernelGGL((m2l_dense_kernel<P, 3>), grid, blk, 0, s, a, M32, L); break;
-    default: hipLaunchKernelGGL((m2l_dense_kernel<P, 4>), grid, blk, 0, s, a, M32, L); break;
-    }
+    if (a.waves == 4) launch_m2l_dense_pw<P, 4>(a, nc, M32, L, s);
+    else launch_m2l_dense_pw<P, 8>(a, nc, M32, L, s);
 }
 
 }  // namespace sfmm

@@ -248,3 +248,26 @@ def test_mixed_every_kind_against_oracle(p):
     err = rel_l2(U, t.fmm(p, q, O.POTGRAD))
     print(f"  potgrad: {err:.3e}", flush=True)
     assert err < 1e-6
+
+
+@pytest.mark.parametrize("precision", [api.FP64, api.MIXED], ids=["fp64", "mixed"])
+def test_tuned_tiles_keep_the_results(torch_dev, precision):
+    # sfmm_plan_tune picks the tile shapes by timing; whatever it picks, the
+    # evaluation stays the method: FP64 against the oracle at 1e-12, mixed within
+    # its rounding of FP64
+    torch, dev = torch_dev
+    pos, q = W.plummer(30000, seed=97)
+    part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, 64)
+    fmm = api.FMM3D(part, 10, ncharge=1, precision=precision)
+    dq = torch.from_numpy(q).to(dev)
+    print(f"tune precision={precision}", flush=True)
+    fmm.tune(api.POTGRAD, 1, dq.data_ptr())
+    tiles = fmm.tiles()
+    assert tiles["p2p_targets_per_lane"] in (1, 2, 4)
+    assert tiles["m2l_waves"] in ((4, 8) if precision == api.MIXED else (0,))
+    U = np.zeros((30000, 4))
+    fmm.rinv_potgrad(q, U)
+    ref = O.Tree(pos, 20, 64).fmm(10, q, O.POTGRAD)
+    err = rel_l2(U, ref)
+    print(f"  tiles {tiles}: rel L2 vs oracle {err:.2e}", flush=True)
+    assert err < (1e-12 if precision == api.FP64 else 1e-6)
