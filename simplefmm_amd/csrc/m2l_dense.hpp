@@ -417,6 +417,12 @@ __global__ __launch_bounds__(64 * NW) void m2l_dense_kernel(DenseArgs a, const f
         // (two arrays, so the reads are seen not to alias the copy in flight)
         auto entry = [&](const dh8* __restrict__ cur, dh8* __restrict__ nxt) {
             const bool more = i + 1 < ne;
+            // the lane group, re-read each entry, so that the per-lane azimuth
+            // factors of the slots shared by orders m and 0 are selected where they
+            // are used instead of being held in registers across the loop (which
+            // spilled the running sums)
+            int qv = q;
+            asm volatile("" : "+v"(qv));
             const int nmeta = more ? smeta[i + 1] : -1;
             // the operator of entry i is in LDS and every wave is done with entry i - 1.
             // After its copy (issued at the top of entry i - 1) an active wave issued a
@@ -449,7 +455,7 @@ __global__ __launch_bounds__(64 * NW) void m2l_dense_kernel(DenseArgs a, const f
                         constexpr int m = dense_slot_m(Cfg::O, s), z = dense_slot_zero(Cfg::O, s);
                         float cm = csx[m], sm = csy[m];
                         if constexpr (z != 0 && m != 0) {  // lane groups whose element is of order 0
-                            const bool zq = (z >> q) & 1;
+                            const bool zq = (z >> qv) & 1;
                             cm = zq ? sc : cm;
                             sm = zq ? 0.0f : sm;
                         }
