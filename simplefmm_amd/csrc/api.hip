@@ -100,6 +100,57 @@ hipError_t dev_malloc(T** p, size_t bytes)
         if (e_ != hipSuccess) return fail(SFMM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// An uninitialised host array: its pages are first touched by the threads that
+// fill it (a zero-initialising std::vector of the plan's sizes spends most of
+// its time in a single-threaded memset).
+template <class T>
+struct HostArray {
+    std::unique_ptr<T[]> data;
+    size_t size = 0;
+    void resize(size_t n)
+    {
+        data.reset(new T[n]);
+        size = n;
+    }
+    T& operator[](size_t i) { return data[i]; }
+    const T& operator[](size_t i) const { return data[i]; }
+};
+
+// Host tables depend only on the expansion order: built once per order per
+// process and shared by every plan (plans copy what they keep).
+struct DenseHost {
+    std::vector<uint16_t> ops;
+    std::vector<float> az;
+    std::vector<int32_t> grp;
+    int op_exp = 0;
+};
+
+std::mutex g_tables_mu;
+std::shared_ptr<const sfmm::Tables> g_tables[17];
+std::shared_ptr<const DenseHost> g_dense[17];
+
+std::shared_ptr<const sfmm::Tables> tables_for(int p)
+{
+    std::lock_guard<std::mutex> lock(g_tables_mu);
+    if (!g_tables[p]) {
+        auto t = std::make_shared<sfmm::Tables>();
+        sfmm::build_tables(p, *t);
+        g_tables[p] = t;
+    }
+    return g_tables[p];
+}
+
+std::shared_ptr<const DenseHost> dense_for(int p, const sfmm::Tables& t)
+{
+    std::lock_guard<std::mutex> lock(g_tables_mu);
+    if (!g_dense[p]) {
+        auto d = std::make_shared<DenseHost>();
+        sfmm::build_dense_m2l(t, d->ops, d->op_exp, d->az, d->grp);
+        g_dense[p] = d;
+    }
+    return g_dense[p];
+}
+
 }  // namespace
 
 struct sfmm_tree_s {
@@ -194,16 +245,26 @@ struct sfmm_plan_s {
     }
 
     template <class T>
-    int upload(const std::vector<T>& v, const T** out)
+    int upload_ptr(const T* v, size_t count, const T** out)
     {
         *out = nullptr;
-        if (v.empty()) return SFMM_OK;
+        if (count == 0) return SFMM_OK;
         void* p = nullptr;
-        if (dev_malloc(&p, v.size() * sizeof(T)) != hipSuccess) return fail(SFMM_ERR_ALLOC, "hipMalloc failed (plan tables)");
+        if (dev_malloc(&p, count * sizeof(T)) != hipSuccess) return fail(SFMM_ERR_ALLOC, "hipMalloc failed (plan tables)");
         owned.push_back(p);
-        HIP_TRY(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(p, v, count * sizeof(T), hipMemcpyHostToDevice));
         *out = static_cast<const T*>(p);
         return SFMM_OK;
+    }
+    template <class T>
+    int upload(const std::vector<T>& v, const T** out)
+    {
+        return upload_ptr(v.data(), v.size(), out);
+    }
+    template <class T>
+    int upload(const HostArray<T>& v, const T** out)
+    {
+        return upload_ptr(v.data.get(), v.size, out);
     }
 };
 
@@ -315,11 +376,10 @@ int ensure_buffers(sfmm_plan_s* P, int nc, int qdim)
 // with meta(offset) and the source of every target (-1: none).  Blocks are cut
 // on all host threads; fails with more than 189 offsets in a block.
 int cut_m2l_blocks(sfmm_plan_s* P, int TB, const std::vector<int>& order, const std::function<int32_t(int)>& meta,
-                   std::vector<int32_t>& blk_tgt, std::vector<int64_t>& blk_row, std::vector<int32_t>& ent_meta,
-                   std::vector<int32_t>& ent_src)
+                   HostArray<int32_t>& blk_tgt, std::vector<int64_t>& blk_row, HostArray<int32_t>& ent_meta,
+                   HostArray<int32_t>& ent_src)
 {
     const sfmm::Tree& t = P->tree->t;
-    blk_row.assign(1, 0);
     struct Blk {
         int level, octant;
         int64_t first;  // index of the first target in the level's octant list
@@ -335,52 +395,75 @@ int cut_m2l_blocks(sfmm_plan_s* P, int TB, const std::vector<int>& order, const 
             for (int o = 0; o < 8; ++o)
                 if (q * TB < oct_nodes[8 * l + o].size()) blocks.push_back({l, o, static_cast<int64_t>(q * TB)});
     }
+    const int64_t nb = static_cast<int64_t>(blocks.size());
     const int nth = host_threads();
-    std::vector<std::vector<int32_t>> t_tgt(nth), t_meta(nth), t_src(nth);
-    std::vector<std::vector<int64_t>> t_cnt(nth);
-    std::vector<char> t_bad(nth, 0);
-    parallel_ranges(static_cast<int64_t>(blocks.size()), nth, [&](int64_t b0, int64_t b1, int c) {
+    std::vector<char> in_order(343, 0);
+    for (int off : order) in_order[off] = 1;
+    // pass 1: the offsets present in each block; pass 2 writes every block's
+    // entries at its place in the final arrays (both over host threads)
+    std::vector<int64_t> cnt(nb, 0);
+    parallel_ranges(nb, nth, [&](int64_t b0, int64_t b1, int) {
+        std::vector<char> used(343);
+        for (int64_t bi = b0; bi < b1; ++bi) {
+            const Blk& B = blocks[bi];
+            const auto& lv = t.lv[B.level];
+            const auto& nodes = oct_nodes[8 * B.level + B.octant];
+            std::fill(used.begin(), used.end(), 0);
+            int64_t c = 0;
+            for (int tt = 0; tt < TB && B.first + tt < static_cast<int64_t>(nodes.size()); ++tt) {
+                const int32_t a = nodes[B.first + tt];
+                for (int64_t k = lv.il_row[a]; k < lv.il_row[a + 1]; ++k) {
+                    const int off = lv.il_off[k];
+                    if (!used[off] && in_order[off]) ++c;
+                    used[off] = 1;
+                }
+            }
+            cnt[bi] = c;
+        }
+    });
+    blk_row.assign(static_cast<size_t>(nb) + 1, 0);
+    for (int64_t b = 0; b < nb; ++b) {
+        if (cnt[b] > 189) return fail(SFMM_ERR_STATE, "M2L block with more than 189 interaction offsets");
+        blk_row[b + 1] = blk_row[b] + cnt[b];
+    }
+    blk_tgt.resize(static_cast<size_t>(nb) * TB);
+    ent_meta.resize(static_cast<size_t>(blk_row[nb]));
+    ent_src.resize(static_cast<size_t>(blk_row[nb]) * TB);
+    parallel_ranges(nb, nth, [&](int64_t b0, int64_t b1, int) {
         std::vector<int32_t> src(static_cast<size_t>(343) * TB);
         std::vector<char> used(343, 0);
         for (int64_t bi = b0; bi < b1; ++bi) {
             const Blk& B = blocks[bi];
             const auto& lv = t.lv[B.level];
             const auto& nodes = oct_nodes[8 * B.level + B.octant];
-            std::fill(src.begin(), src.end(), -1);
             for (int tt = 0; tt < TB; ++tt) {
                 if (B.first + tt >= static_cast<int64_t>(nodes.size())) {
-                    t_tgt[c].push_back(-1);
+                    blk_tgt[bi * TB + tt] = -1;
                     continue;
                 }
                 const int32_t a = nodes[B.first + tt];
-                t_tgt[c].push_back(static_cast<int32_t>(P->level_first[B.level] + a));
+                blk_tgt[bi * TB + tt] = static_cast<int32_t>(P->level_first[B.level] + a);
                 for (int64_t k = lv.il_row[a]; k < lv.il_row[a + 1]; ++k) {
                     const int off = lv.il_off[k];
-                    src[static_cast<size_t>(off) * TB + tt] = static_cast<int32_t>(P->level_first[B.level] + lv.il_src[k]);
-                    used[off] = 1;
+                    int32_t* row = src.data() + static_cast<size_t>(off) * TB;
+                    if (!used[off]) {
+                        std::fill(row, row + TB, -1);
+                        used[off] = 1;
+                    }
+                    row[tt] = static_cast<int32_t>(P->level_first[B.level] + lv.il_src[k]);
                 }
             }
-            int64_t cnt = 0;
+            int64_t e = blk_row[bi];
             for (int off : order) {
                 if (!used[off]) continue;
-                used[off] = 0;
-                t_meta[c].push_back(meta(off));
-                t_src[c].insert(t_src[c].end(), src.begin() + static_cast<size_t>(off) * TB,
-                                src.begin() + static_cast<size_t>(off + 1) * TB);
-                ++cnt;
+                ent_meta[e] = meta(off);
+                std::copy(src.begin() + static_cast<size_t>(off) * TB, src.begin() + static_cast<size_t>(off + 1) * TB,
+                          ent_src.data.get() + e * TB);
+                ++e;
             }
-            if (cnt > 189) t_bad[c] = 1;
-            t_cnt[c].push_back(cnt);
+            for (int off = 0; off < 343; ++off) used[off] = 0;
         }
     });
-    for (char bad : t_bad)
-        if (bad) return fail(SFMM_ERR_STATE, "M2L block with more than 189 interaction offsets");
-    for (int c = 0; c < nth; ++c) {
-        blk_tgt.insert(blk_tgt.end(), t_tgt[c].begin(), t_tgt[c].end());
-        ent_meta.insert(ent_meta.end(), t_meta[c].begin(), t_meta[c].end());
-        ent_src.insert(ent_src.end(), t_src[c].begin(), t_src[c].end());
-        for (int64_t v : t_cnt[c]) blk_row.push_back(blk_row.back() + v);
-    }
     return SFMM_OK;
 }
 
@@ -401,7 +484,7 @@ int build_dense_blocks(sfmm_plan_s* P, int nc)
     std::sort(order.begin(), order.end(), [&](int x, int y) {
         return grp[x] != grp[y] ? grp[x] < grp[y] : cls[x] != cls[y] ? cls[x] < cls[y] : x < y;
     });
-    std::vector<int32_t> blk_tgt, ent_meta, ent_src;
+    HostArray<int32_t> blk_tgt, ent_meta, ent_src;
     std::vector<int64_t> blk_row;
     if (int rc = cut_m2l_blocks(P, TB, order, [&](int off) { return cls[off] | grp[off] << 8; }, blk_tgt, blk_row,
                                 ent_meta, ent_src))
@@ -451,7 +534,7 @@ int build_m2l_blocks(sfmm_plan_s* P, int nc)
     std::sort(order.begin(), order.end(), [&](int x, int y) {
         return cls[x] != cls[y] ? cls[x] < cls[y] : d2[x] != d2[y] ? d2[x] < d2[y] : x < y;
     });
-    std::vector<int32_t> blk_tgt, ent_meta, ent_src;
+    HostArray<int32_t> blk_tgt, ent_meta, ent_src;
     std::vector<int64_t> blk_row;
     if (int rc = cut_m2l_blocks(P, TB, order, [&](int off) { return off | cls[off] << 9 | d2[off] << 16; }, blk_tgt,
                                 blk_row, ent_meta, ent_src))
@@ -1033,7 +1116,7 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
     for (auto& e : P->ev) HIP_TRY(hipEventCreate(&e));
     const sfmm::Tree& t = tree->t;
     const int p = P->p, K = P->K;
-    sfmm::build_tables(p, P->tables);
+    P->tables = *tables_for(p);
     const sfmm::Tables& tb = P->tables;
 
     // ---- nodes, numbered level by level (gnode)
@@ -1055,8 +1138,10 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
     }
     // ---- particles in sorted order
     const int64_t n = t.n;
-    std::vector<double> px(n), py(n), pz(n), ux(n, 0.0), uy(n, 0.0), uz(n, 0.0);
-    std::vector<int32_t> iperm(n);
+    HostArray<double> px, py, pz, ux, uy, uz;
+    HostArray<int32_t> iperm;
+    for (auto* a : {&px, &py, &pz, &ux, &uy, &uz}) a->resize(n);
+    iperm.resize(n);
     const int nthreads = host_threads();
     parallel_ranges(n, nthreads, [&](int64_t s0, int64_t s1, int) {
         for (int64_t s = s0; s < s1; ++s) {
@@ -1064,6 +1149,7 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
             px[s] = x[0];
             py[s] = x[1];
             pz[s] = x[2];
+            ux[s] = uy[s] = uz[s] = 0.0;  // set below for particles of far leaves
             iperm[t.perm[s]] = static_cast<int32_t>(s);
         }
     });
@@ -1267,9 +1353,11 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
         return rc;
     P->dense = P->precision == SFMM_MIXED && P->m2l_kernel == 0 && p <= sfmm::kDenseMaxP;
     if (P->dense) {
-        std::vector<uint16_t> ops;
-        std::vector<float> az;
-        sfmm::build_dense_m2l(tb, ops, P->dense_op_exp, az, P->dense_grp);
+        const std::shared_ptr<const DenseHost> dh = dense_for(p, tb);
+        const std::vector<uint16_t>& ops = dh->ops;
+        const std::vector<float>& az = dh->az;
+        P->dense_op_exp = dh->op_exp;
+        P->dense_grp = dh->grp;
         P->dense_kd = sfmm::dense_order(p).kd;
         const uint16_t* dops = nullptr;
         const float* dmax = nullptr;

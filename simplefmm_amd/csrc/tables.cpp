@@ -10,10 +10,32 @@
 #include "m2l_sched.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
+#include <thread>
 
 namespace sfmm {
+
+namespace {
+
+// f(k) for k in [0, n) on up to 16 host threads (each k is independent).
+template <class F>
+void par_for(int n, F&& f)
+{
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = std::max(1, std::min(n, static_cast<int>(std::min(16u, hw))));
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int k = next++; k < n; k = next++) f(k);
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nt; ++i) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+}
+
+}  // namespace
 
 namespace {
 
@@ -210,7 +232,7 @@ void build_tables(int p, Tables& t)
     //      (sqrt .75)^n 2^-j Zc(n,-m) e^{-i m phi_c} M~c(j-n,k-m)
     t.m2m.resize(static_cast<size_t>(8) * K * K);
     t.l2l.resize(static_cast<size_t>(8) * K * K);
-    for (int cc = 0; cc < 8; ++cc) {
+    par_for(8, [&](int cc) {
         std::vector<double> Tre(static_cast<size_t>(K) * K, 0.0), Tim(static_cast<size_t>(K) * K, 0.0);
         for (int j = 0; j <= p; ++j)
             for (int k = -j; k <= j; ++k)
@@ -247,7 +269,7 @@ void build_tables(int p, Tables& t)
                     }
         complex_to_real_form(p, Tre, Tim, R);
         std::copy(R.begin(), R.end(), t.l2l.begin() + static_cast<size_t>(cc) * K * K);
-    }
+    });
 
     // ---- rotations for the 70 polar classes (t = di^2+dj^2 in tvals, dk in [-3,3])
     static const int tvals[10] = {0, 1, 2, 4, 5, 8, 9, 10, 13, 18};
@@ -256,9 +278,10 @@ void build_tables(int p, Tables& t)
     for (int n = 0; n <= p; ++n) t.rot_off[n + 1] = t.rot_off[n] + (n + 1) * (n + 1) + n * n;
     t.rot_size = t.rot_off[p + 1];
     t.rot.assign(static_cast<size_t>(70) * 2 * t.rot_size, 0.0);
-    for (int ti = 0; ti < 10; ++ti)
-        for (int dk = -3; dk <= 3; ++dk) {
-            if (tvals[ti] == 0 && dk == 0) continue;
+    par_for(70, [&](int cls) {
+        const int ti = cls / 7, dk = cls % 7 - 3;
+        {
+            if (tvals[ti] == 0 && dk == 0) return;
             const long double theta = std::acos(dk / std::sqrt(static_cast<long double>(tvals[ti] + dk * dk)));
             double* fwd = t.rot.data() + static_cast<size_t>(7 * ti + dk + 3) * 2 * t.rot_size;
             double* bwd = fwd + t.rot_size;
@@ -286,6 +309,7 @@ void build_tables(int p, Tables& t)
                     }
             }
         }
+    });
 
     // ---- coaxial translation blocks for |d|^2 = 0..27
     t.coax_off.resize(p + 2);
@@ -386,9 +410,9 @@ void build_dense_m2l(const Tables& t, std::vector<uint16_t>& ops, int& op_exp, s
     const int nrow = 16 * O.mtc, ncol = O.kc;
     const size_t per_part = static_cast<size_t>(nrow) * ncol;
     std::vector<double> D(70 * 2 * per_part, 0.0);
-    double dmax = 0.0;
-    for (int c = 0; c < 70; ++c) {
-        if (cls_d2[c] < 1) continue;
+    std::vector<double> cmax(70, 0.0);
+    par_for(70, [&](int c) {
+        if (cls_d2[c] < 1) return;
         const double* fwd = t.rot.data() + static_cast<size_t>(c) * 2 * t.rot_size;
         const double* bwd = fwd + t.rot_size;
         const double* C = t.coax.data() + static_cast<size_t>(cls_d2[c]) * t.coax_size;
@@ -413,11 +437,12 @@ void build_dense_m2l(const Tables& t, std::vector<uint16_t>& ops, int& op_exp, s
                                  fwd[t.rot_off[n] + (n + 1) * (n + 1) + (q - 1) * n + (b - 1)];
                     }
                     Dp[static_cast<size_t>(kr) * ncol + kk] = v;
-                    dmax = std::max(dmax, std::fabs(v));
+                    cmax[c] = std::max(cmax[c], std::fabs(v));
                 }
             }
         }
-    }
+    });
+    const double dmax = *std::max_element(cmax.begin(), cmax.end());
     int e = 0;
     if (dmax > 0.0) (void)std::frexp(dmax, &e);  // dmax < 2^e
     op_exp = 14 - e;
@@ -426,7 +451,7 @@ void build_dense_m2l(const Tables& t, std::vector<uint16_t>& ops, int& op_exp, s
     // cos fragment mt * nks + ks, sin fragment (mtc + mt) * nks + ks
     const size_t per_class = static_cast<size_t>(O.chunks) * 512;  // padded to whole copy chunks
     ops.assign(per_class * 70, 0);
-    for (int c = 0; c < 70; ++c)
+    par_for(70, [&](int c) {
         for (int part = 0; part < 2; ++part) {
             const double* Dp = D.data() + (static_cast<size_t>(c) * 2 + part) * per_part;
             const int mts = part == 0 ? O.mtc : O.mts;
@@ -445,6 +470,7 @@ void build_dense_m2l(const Tables& t, std::vector<uint16_t>& ops, int& op_exp, s
                         }
                 }
         }
+    });
     // azimuth groups: the primitive directions (a, b) of (di, dj), (0, 0) with (1, 0)
     std::vector<std::pair<int, int>> dirs;
     off_grp.assign(343, -1);
