@@ -1136,23 +1136,9 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
             child0[g] = lv.child0[a] >= 0 ? static_cast<int32_t>(P->level_first[l + 1] + lv.child0[a]) : -1;
         }
     }
-    // ---- particles in sorted order
+    // ---- leaves, and the leaves with far field (levels >= 2) with their centres
     const int64_t n = t.n;
-    HostArray<double> px, py, pz, ux, uy, uz;
-    HostArray<int32_t> iperm;
-    for (auto* a : {&px, &py, &pz, &ux, &uy, &uz}) a->resize(n);
-    iperm.resize(n);
     const int nthreads = host_threads();
-    parallel_ranges(n, nthreads, [&](int64_t s0, int64_t s1, int) {
-        for (int64_t s = s0; s < s1; ++s) {
-            const double* x = tree->pos.data() + 3 * static_cast<int64_t>(t.perm[s]);
-            px[s] = x[0];
-            py[s] = x[1];
-            pz[s] = x[2];
-            ux[s] = uy[s] = uz[s] = 0.0;  // set below for particles of far leaves
-            iperm[t.perm[s]] = static_cast<int32_t>(s);
-        }
-    });
     std::vector<int32_t> fl_gnode, fl_level;
     std::vector<int64_t> fl_begin, fl_end, lf_begin, lf_end;
     for (size_t g = 0; g < t.leaf_node.size(); ++g) {
@@ -1167,26 +1153,49 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
         fl_begin.push_back(lv.pbegin[a]);
         fl_end.push_back(lv.pend[a]);
     }
-    // particle coordinates relative to their far leaf's centre, in leaf widths
-    parallel_ranges(static_cast<int64_t>(fl_gnode.size()), nthreads, [&](int64_t f0, int64_t f1, int) {
+    const int64_t nfl = static_cast<int64_t>(fl_gnode.size());
+    std::vector<double> fl_c(3 * static_cast<size_t>(nfl)), fl_iw(nfl);
+    HostArray<int32_t> slot_fleaf;  // far leaf of each slot, -1 for none
+    slot_fleaf.resize(n);
+    parallel_ranges(n, nthreads, [&](int64_t s0, int64_t s1, int) {
+        std::fill(slot_fleaf.data.get() + s0, slot_fleaf.data.get() + s1, -1);
+    });
+    parallel_ranges(nfl, nthreads, [&](int64_t f0, int64_t f1, int) {
         for (int64_t f = f0; f < f1; ++f) {
             const int l = fl_level[f];
             const int32_t a = static_cast<int32_t>(fl_gnode[f] - P->level_first[l]);
-            double c[3];
-            t.box.centre(l, t.lv[l].code[a], c);
-            const double iw = std::ldexp(1.0, l) / t.box.width;
-            for (int64_t s = fl_begin[f]; s < fl_end[f]; ++s) {
-                double r[3];
-                t.box.rotate(tree->pos.data() + 3 * static_cast<int64_t>(t.perm[s]), r);
-                ux[s] = (r[0] - c[0]) * iw;
-                uy[s] = (r[1] - c[1]) * iw;
-                uz[s] = (r[2] - c[2]) * iw;
-            }
+            t.box.centre(l, t.lv[l].code[a], &fl_c[3 * f]);
+            fl_iw[f] = std::ldexp(1.0, l) / t.box.width;
+            for (int64_t s = fl_begin[f]; s < fl_end[f]; ++s) slot_fleaf[s] = static_cast<int32_t>(f);
         }
     });
-    std::vector<int32_t> slot_fleaf(n, -1);
-    for (size_t f = 0; f < fl_begin.size(); ++f)
-        for (int64_t sl = fl_begin[f]; sl < fl_end[f]; ++sl) slot_fleaf[sl] = static_cast<int32_t>(f);
+    // ---- particles in sorted order, in one pass: positions, the inverse
+    // permutation, and coordinates relative to their far leaf's centre in leaf widths
+    HostArray<double> px, py, pz, ux, uy, uz;
+    HostArray<int32_t> iperm;
+    for (auto* a : {&px, &py, &pz, &ux, &uy, &uz}) a->resize(n);
+    iperm.resize(n);
+    parallel_ranges(n, nthreads, [&](int64_t s0, int64_t s1, int) {
+        for (int64_t s = s0; s < s1; ++s) {
+            const double* x = tree->pos.data() + 3 * static_cast<int64_t>(t.perm[s]);
+            px[s] = x[0];
+            py[s] = x[1];
+            pz[s] = x[2];
+            iperm[t.perm[s]] = static_cast<int32_t>(s);
+            const int32_t f = slot_fleaf[s];
+            if (f < 0) {
+                ux[s] = uy[s] = uz[s] = 0.0;
+                continue;
+            }
+            double r[3];
+            t.box.rotate(x, r);
+            const double* c = &fl_c[3 * static_cast<size_t>(f)];
+            const double iw = fl_iw[f];
+            ux[s] = (r[0] - c[0]) * iw;
+            uy[s] = (r[1] - c[1]) * iw;
+            uz[s] = (r[2] - c[2]) * iw;
+        }
+    });
     // ---- self lists (fmm.hpp:398-408): per target, listed sources other than
     // itself, flagged when in its near field (is_neighbor, partition.hpp:484-488)
     std::vector<int64_t> sc_tgt, sc_row{0}, sc_src;
@@ -1254,35 +1263,57 @@ int sfmm_plan_create(sfmm_tree tree, const sfmm_plan_desc* desc, sfmm_plan* out)
         }
     }
     // ---- near-field work items: a leaf's concatenated near sources cut into runs
-    //      of at most ~2^18 pairs; split leaves go through deterministic partial sums
+    //      of at most ~2^18 pairs; split leaves go through deterministic partial sums.
+    //      Per leaf (host threads): its source count and run length; then the item
+    //      and partial-row offsets by prefix sums; then the items in place.
+    const int64_t nleaf = static_cast<int64_t>(t.leaf_node.size());
     std::vector<int64_t> near_pref(t.near_begin.size());
-    std::vector<int64_t> it_leaf, it_begin, it_count, it_part, sp_slot, sp_first, sp_nitem, sp_stride;
-    int64_t prow = 0;
-    for (size_t g = 0; g < t.leaf_node.size(); ++g) {
-        int64_t total = 0;
-        for (int64_t r = t.near_row[g]; r < t.near_row[g + 1]; ++r) {
-            near_pref[r] = total;
-            total += t.near_end[r] - t.near_begin[r];
-        }
-        const int64_t nt = lf_end[g] - lf_begin[g];
-        const int64_t per = std::max<int64_t>(64, ((int64_t(1) << 18) / std::max<int64_t>(nt, 1)) / 64 * 64);
-        const int64_t nitem = std::max<int64_t>(1, (total + per - 1) / per);
-        for (int64_t i = 0; i < nitem; ++i) {
-            it_leaf.push_back(static_cast<int64_t>(g));
-            it_begin.push_back(i * per);
-            it_count.push_back(std::min(per, total - i * per));
-            it_part.push_back(nitem == 1 ? -1 : prow + i * nt);
-        }
-        if (nitem > 1) {
-            for (int64_t k = 0; k < nt; ++k) {
-                sp_slot.push_back(lf_begin[g] + k);
-                sp_first.push_back(prow + k);
-                sp_nitem.push_back(nitem);
-                sp_stride.push_back(nt);
+    std::vector<int64_t> l_total(nleaf), l_per(nleaf), l_nitem(nleaf), l_item0(nleaf + 1, 0), l_row0(nleaf + 1, 0),
+        l_split0(nleaf + 1, 0);
+    parallel_ranges(nleaf, nthreads, [&](int64_t g0, int64_t g1, int) {
+        for (int64_t g = g0; g < g1; ++g) {
+            int64_t total = 0;
+            for (int64_t r = t.near_row[g]; r < t.near_row[g + 1]; ++r) {
+                near_pref[r] = total;
+                total += t.near_end[r] - t.near_begin[r];
             }
-            prow += nitem * nt;
+            const int64_t nt = lf_end[g] - lf_begin[g];
+            const int64_t per = std::max<int64_t>(64, ((int64_t(1) << 18) / std::max<int64_t>(nt, 1)) / 64 * 64);
+            l_per[g] = per;
+            l_nitem[g] = std::max<int64_t>(1, (total + per - 1) / per);
+            l_total[g] = total;
         }
+    });
+    for (int64_t g = 0; g < nleaf; ++g) {
+        const int64_t nt = lf_end[g] - lf_begin[g];
+        const bool split = l_nitem[g] > 1;
+        l_item0[g + 1] = l_item0[g] + l_nitem[g];
+        l_row0[g + 1] = l_row0[g] + (split ? l_nitem[g] * nt : 0);
+        l_split0[g + 1] = l_split0[g] + (split ? nt : 0);
     }
+    const int64_t nitems = l_item0[nleaf], nsplit = l_split0[nleaf], prow = l_row0[nleaf];
+    std::vector<int64_t> it_leaf(nitems), it_begin(nitems), it_count(nitems), it_part(nitems);
+    std::vector<int64_t> sp_slot(nsplit), sp_first(nsplit), sp_nitem(nsplit), sp_stride(nsplit);
+    parallel_ranges(nleaf, nthreads, [&](int64_t g0, int64_t g1, int) {
+        for (int64_t g = g0; g < g1; ++g) {
+            const int64_t nt = lf_end[g] - lf_begin[g], per = l_per[g], nitem = l_nitem[g], total = l_total[g];
+            for (int64_t i = 0; i < nitem; ++i) {
+                const int64_t k = l_item0[g] + i;
+                it_leaf[k] = g;
+                it_begin[k] = i * per;
+                it_count[k] = std::min(per, total - i * per);
+                it_part[k] = nitem == 1 ? -1 : l_row0[g] + i * nt;
+            }
+            if (nitem > 1)
+                for (int64_t j = 0; j < nt; ++j) {
+                    const int64_t k = l_split0[g] + j;
+                    sp_slot[k] = lf_begin[g] + j;
+                    sp_first[k] = l_row0[g] + j;
+                    sp_nitem[k] = nitem;
+                    sp_stride[k] = nt;
+                }
+        }
+    });
     P->p2p.n_item = static_cast<int64_t>(it_leaf.size());
     P->p2p.n_split_targets = static_cast<int64_t>(sp_slot.size());
     P->p2p.n_partial_rows = prow;
