@@ -144,6 +144,9 @@ int sfmm_plan_stream(sfmm_plan plan, void** stream);
 int sfmm_plan_tune(sfmm_plan plan, int kind, int ncharge, const double* dQ);
 /* The plan's tile shapes (m2l_waves 0: the plan does not use the dense M2L). */
 int sfmm_plan_tiles(sfmm_plan plan, int* p2p_targets_per_lane, int* m2l_waves);
+/* Sets the tile shapes explicitly (p2p 1, 2 or 4; m2l 4 or 8, or 0 to keep; the
+ * M2L shape applies to plans that use the dense M2L).  Waits for the plan's stream. */
+int sfmm_plan_set_tiles(sfmm_plan plan, int p2p_targets_per_lane, int m2l_waves);
 int sfmm_synchronize(sfmm_plan plan);
 
 /* Device milliseconds of each stage of the last evaluation:

@@ -82,6 +82,7 @@ SIGNATURES = {
     "sfmm_plan_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "sfmm_plan_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "sfmm_plan_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "sfmm_plan_set_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfmm_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "sfmm_last_stage_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "sfmm_record_stages": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -312,6 +313,10 @@ class FMM3D:
     def tune(self, kind: int, ncharge: int, q_ptr: int):
         """Times the kernels' tile shapes with these device charges and keeps the fastest."""
         _check(lib().sfmm_plan_tune(self.h, kind, ncharge, ctypes.c_void_p(q_ptr)))
+
+    def set_tiles(self, p2p_targets_per_lane: int, m2l_waves: int = 0):
+        """Sets the tile shapes (near field 1, 2 or 4 targets per lane; dense M2L 4 or 8 wavefronts, 0 keeps)."""
+        _check(lib().sfmm_plan_set_tiles(self.h, int(p2p_targets_per_lane), int(m2l_waves)))
 
     def tiles(self) -> dict:
         """{'p2p_targets_per_lane': T, 'm2l_waves': W} (W = 0 without the dense M2L)."""

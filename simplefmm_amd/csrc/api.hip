@@ -1498,6 +1498,20 @@ int sfmm_plan_tune(sfmm_plan plan, int kind, int ncharge, const double* dQ)
     return SFMM_OK;
 }
 
+int sfmm_plan_set_tiles(sfmm_plan plan, int p2p_targets_per_lane, int m2l_waves)
+{
+    if (!plan) return fail(SFMM_ERR_ARG, "null argument");
+    if (p2p_targets_per_lane != 1 && p2p_targets_per_lane != 2 && p2p_targets_per_lane != 4)
+        return fail(SFMM_ERR_ARG, "near-field targets per lane must be 1, 2 or 4");
+    if (m2l_waves != 0 && m2l_waves != 4 && m2l_waves != 8)
+        return fail(SFMM_ERR_ARG, "M2L wavefronts per workgroup must be 4 or 8 (0: keep)");
+    HIP_TRY(hipSetDevice(plan->device));
+    HIP_TRY(hipStreamSynchronize(plan->stream));  // work in flight keeps its shapes
+    plan->p2p_tpl = p2p_targets_per_lane;
+    if (m2l_waves != 0 && plan->dense) set_dense_waves(plan, m2l_waves);
+    return SFMM_OK;
+}
+
 int sfmm_plan_tiles(sfmm_plan plan, int* p2p_targets_per_lane, int* m2l_waves)
 {
     if (!plan || !p2p_targets_per_lane || !m2l_waves) return fail(SFMM_ERR_ARG, "null argument");

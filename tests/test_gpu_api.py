@@ -271,3 +271,29 @@ def test_tuned_tiles_keep_the_results(torch_dev, precision):
     err = rel_l2(U, ref)
     print(f"  tiles {tiles}: rel L2 vs oracle {err:.2e}", flush=True)
     assert err < (1e-12 if precision == api.FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("tpl,waves", [(1, 8), (4, 8), (2, 4), (1, 4)])
+def test_every_tile_shape_against_oracle(torch_dev, tpl, waves):
+    # every near-field and dense-M2L tile shape the tuner can pick: FP64 against the
+    # oracle at 1e-12, mixed within its rounding, for potential+gradient and a
+    # three-component potential
+    torch, dev = torch_dev
+    pos, q = W.clusters(12000, seed=98)
+    part = api.OctreePartitioner(pos, api.Box(pos)).refine(20, 40)
+    tree = O.Tree(pos, 20, 40)
+    Q3 = np.random.default_rng(5).uniform(-1, 1, (12000, 3))
+    ref_pg = tree.fmm(8, q, O.POTGRAD)
+    ref_p3 = np.stack([tree.fmm(8, Q3[:, c].copy(), O.POT) for c in range(3)], axis=1)
+    for prec in (api.FP64, api.MIXED):
+        fmm = api.FMM3D(part, 8, ncharge=3, precision=prec)
+        fmm.set_tiles(tpl, waves)
+        assert fmm.tiles()["p2p_targets_per_lane"] == tpl
+        print(f"tiles tpl={tpl} waves={waves} precision={prec}", flush=True)
+        U = np.zeros((12000, 4))
+        fmm.rinv_potgrad(q, U)
+        V = np.zeros((12000, 3))
+        fmm._eval(api.POTENTIAL, 3, Q3, V, api.BOTH)
+        tol = 1e-12 if prec == api.FP64 else 2e-6
+        assert rel_l2(U, ref_pg) < tol
+        assert rel_l2(V, ref_p3) < tol
