@@ -236,57 +236,64 @@ __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int nc, co
 
 // ------------------------------------------------------------------ L2P
 // One thread per particle of the far-field leaves (slot_fleaf maps a sorted
-// slot to its leaf): U_c += (1/w) sum_{n,m} Re(L~ conj R(n,m)), the m > 0 terms
-// doubled for the conjugate half (fmm.hpp:747-771).  The leaf's coefficients
-// are read through L1 (the lanes of a wavefront share a few leaves).  FP64 on
-// both precision paths (FP32 harmonics here cost 3x in potential error for a
-// few percent of the stage).
-template <int P>
-__global__ __launch_bounds__(256) void l2p_kernel(DevPlan d, int nc, const double* __restrict__ L,
-                                                  double* __restrict__ out)
+// slot to its leaf): U_c += (1/w) sum_{n,m} Re(L~ conj R(n,m)), the m > 0
+// terms doubled for the conjugate half (fmm.hpp:747-771).  The harmonics are
+// advanced degree by degree with every azimuth's recurrence carried along, so
+// that the leaf's coefficients are read row by row in memory order (each cache
+// line in one stretch; azimuth-major order revisits every line P+1 times and
+// is 1.5x slower); the factor 2 enters once, in R(1,1), and is exact.  The
+// coefficients reach each lane through the vector L1, whose return rate bounds
+// the kernel.  FP64 on both precision paths (FP32 harmonics here cost 3x in
+// potential error for a few percent of the stage).
+template <int P, int NC>
+__global__ __launch_bounds__(256, P <= 12 ? 4 : 2) void l2p_kernel(DevPlan d, const double* __restrict__ L,
+                                                                   double* __restrict__ out)
 {
-    using R = double;
     constexpr int K = (P + 1) * (P + 1);
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (i >= d.n) return;
     const int32_t f = d.slot_fleaf[i];
     if (f < 0) return;  // near-field-only leaf (level < 2)
     const double invw = d.inv_width[d.fl_level[f]];
-    const double* Lg = L + static_cast<int64_t>(d.fl_gnode[f]) * nc * K;
+    const double* Lg = L + static_cast<int64_t>(d.fl_gnode[f]) * NC * K;
     const double* rec = d.rec;
-    const R x = static_cast<R>(d.ux[i]), y = static_cast<R>(d.uy[i]), z = static_cast<R>(d.uz[i]);
-    const R r2 = x * x + y * y + z * z;
-    double U[4] = {0.0, 0.0, 0.0, 0.0};
-    R dr = 1, di = 0;
+    const double x = d.ux[i], y = d.uy[i], z = d.uz[i];
+    const double r2 = x * x + y * y + z * z;
+    // R(n-1, m) and R(n-2, m) of the degree reached, per azimuth m
+    double cr[P + 1], ci[P + 1], pr[P + 1], pi[P + 1];
 #pragma unroll
-    for (int m = 0; m <= P; ++m) {
-        if (m > 0) {
-            const R c = static_cast<R>(rec_d(rec, m));
-            const R tr = c * (-x * dr - y * di);
-            const R ti = c * (y * dr - x * di);
-            dr = tr;
-            di = ti;
-        }
-        R u[4] = {0, 0, 0, 0};
-        R p1r = dr, p1i = di, p2r = 0, p2i = 0;
-        for (int c = 0; c < nc; ++c)
-            u[c] += static_cast<R>(Lg[c * K + rfc(m, m)]) * p1r + (m > 0 ? static_cast<R>(Lg[c * K + rfs(m, m)]) * p1i : R(0));
+    for (int m = 0; m <= P; ++m) cr[m] = ci[m] = pr[m] = pi[m] = 0.0;
+    cr[0] = 1.0;
+    double U[NC];
 #pragma unroll
-        for (int n = m + 1; n <= P; ++n) {
-            const R a = static_cast<R>(rec_a(rec, n, m)) * z, bb = static_cast<R>(rec_b(rec, n, m)) * r2;
-            const R rr = a * p1r - bb * p2r;
-            const R ri = a * p1i - bb * p2i;
-            p2r = p1r;
-            p2i = p1i;
-            p1r = rr;
-            p1i = ri;
-            for (int c = 0; c < nc; ++c)
-                u[c] += static_cast<R>(Lg[c * K + rfc(n, m)]) * rr + (m > 0 ? static_cast<R>(Lg[c * K + rfs(n, m)]) * ri : R(0));
+    for (int c = 0; c < NC; ++c) U[c] = Lg[c * K];
+#pragma unroll
+    for (int n = 1; n <= P; ++n) {
+        const double tr = cr[n - 1], ti = ci[n - 1];
+#pragma unroll
+        for (int m = 0; m < n; ++m) {
+            const double a = rec_a(rec, n, m) * z, bb = rec_b(rec, n, m) * r2;
+            const double nr = a * cr[m] - bb * pr[m];
+            const double ni = a * ci[m] - bb * pi[m];
+            pr[m] = cr[m];
+            pi[m] = ci[m];
+            cr[m] = nr;
+            ci[m] = ni;
         }
-        const double fm = (m == 0) ? 1.0 : 2.0;
-        for (int c = 0; c < nc; ++c) U[c] += fm * static_cast<double>(u[c]);
+        const double dd = (n == 1 ? 2.0 : 1.0) * rec_d(rec, n);
+        cr[n] = dd * (-x * tr - y * ti);
+        ci[n] = dd * (y * tr - x * ti);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+#pragma unroll
+            for (int m = 0; m <= n; ++m) {
+                U[c] = fma(Lg[c * K + rfc(n, m)], cr[m], U[c]);
+                if (m > 0) U[c] = fma(Lg[c * K + rfs(n, m)], ci[m], U[c]);
+            }
+        }
     }
-    for (int c = 0; c < nc; ++c) out[c * d.n + i] += invw * U[c];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) out[c * d.n + i] += invw * U[c];
 }
 
 // ------------------------------------------------------------------ M2M / L2L
@@ -881,7 +888,13 @@ template <int P>
 struct L2PRun {
     static void run(const DevPlan& d, int nc, const double* L, double* out, hipStream_t s)
     {
-        hipLaunchKernelGGL(l2p_kernel<P>, dim3(grid_for(d.n, 256)), dim3(256), 0, s, d, nc, L, out);
+        const dim3 grid(grid_for(d.n, 256)), blk(256);
+        switch (nc) {
+        case 1: hipLaunchKernelGGL((l2p_kernel<P, 1>), grid, blk, 0, s, d, L, out); break;
+        case 2: hipLaunchKernelGGL((l2p_kernel<P, 2>), grid, blk, 0, s, d, L, out); break;
+        case 3: hipLaunchKernelGGL((l2p_kernel<P, 3>), grid, blk, 0, s, d, L, out); break;
+        default: hipLaunchKernelGGL((l2p_kernel<P, 4>), grid, blk, 0, s, d, L, out); break;
+        }
     }
 };
 template <int P>
