@@ -553,16 +553,16 @@ int build_m2l_blocks(sfmm_plan_s* P, int nc)
     return SFMM_OK;
 }
 
-// M2M blocks: parents of one level with children, 16/nc per block, one entry
-// per child octant present.  L2L blocks: nodes of one level grouped by octant,
-// 16/nc per block, one entry (the octant; inputs are the parents).
+// M2M and L2L work lists of every level: blocks of 16/nc parents, each with one
+// entry per child octant present (the children of that octant).  M2M at parent
+// level l and L2L into child level l+1 run over the same blocks.
 int build_translate_blocks(sfmm_plan_s* P, int nc)
 {
     if (P->m2m[nc].blk_row) return SFMM_OK;
     const sfmm::Tree& t = P->tree->t;
     const int TB = 16 / nc;
-    std::vector<int64_t> row{0}, lrow{0};
-    std::vector<int32_t> out, oct, in, lout, loct, lin;
+    std::vector<int64_t> row{0};
+    std::vector<int32_t> parent_of_block, oct, child;
     auto& lv_m = P->m2m_lvl[nc];
     auto& lv_l = P->l2l_lvl[nc];
     lv_m.assign(t.depth + 2, 0);
@@ -570,66 +570,45 @@ int build_translate_blocks(sfmm_plan_s* P, int nc)
     for (int l = 0; l <= t.depth; ++l) {
         const auto& lv = t.lv[l];
         lv_m[l] = static_cast<int64_t>(row.size()) - 1;
-        if (l < t.depth) {
-            std::vector<int32_t> parents;
-            for (int32_t a = 0; a < lv.n_node(); ++a)
-                if (lv.nchild[a] > 0) parents.push_back(a);
-            for (size_t b = 0; b < parents.size(); b += TB) {
-                int32_t child[8][16];
-                for (auto& r : child) std::fill(r, r + 16, -1);
-                bool present[8] = {};
-                for (int tt = 0; tt < TB; ++tt) {
-                    if (b + tt >= parents.size()) {
-                        out.push_back(-1);
-                        continue;
-                    }
-                    const int32_t a = parents[b + tt];
-                    out.push_back(static_cast<int32_t>(P->level_first[l] + a));
-                    const auto& cl = t.lv[l + 1];
-                    for (int c = 0; c < lv.nchild[a]; ++c) {
-                        const int32_t ci = lv.child0[a] + c;
-                        const int o = static_cast<int>(cl.code[ci] & 7);
-                        child[o][tt] = static_cast<int32_t>(P->level_first[l + 1] + ci);
-                        present[o] = true;
-                    }
+        if (l == t.depth) continue;
+        std::vector<int32_t> parents;
+        for (int32_t a = 0; a < lv.n_node(); ++a)
+            if (lv.nchild[a] > 0) parents.push_back(a);
+        for (size_t b = 0; b < parents.size(); b += TB) {
+            int32_t ch[8][16];
+            for (auto& r : ch) std::fill(r, r + 16, -1);
+            bool present[8] = {};
+            for (int tt = 0; tt < TB; ++tt) {
+                if (b + tt >= parents.size()) {
+                    parent_of_block.push_back(-1);
+                    continue;
                 }
-                for (int o = 0; o < 8; ++o) {
-                    if (!present[o]) continue;
-                    oct.push_back(o);
-                    in.insert(in.end(), child[o], child[o] + TB);
+                const int32_t a = parents[b + tt];
+                parent_of_block.push_back(static_cast<int32_t>(P->level_first[l] + a));
+                const auto& cl = t.lv[l + 1];
+                for (int c = 0; c < lv.nchild[a]; ++c) {
+                    const int32_t ci = lv.child0[a] + c;
+                    const int o = static_cast<int>(cl.code[ci] & 7);
+                    ch[o][tt] = static_cast<int32_t>(P->level_first[l + 1] + ci);
+                    present[o] = true;
                 }
-                row.push_back(static_cast<int64_t>(oct.size()));
             }
-        }
-        lv_l[l] = static_cast<int64_t>(lrow.size()) - 1;
-        if (l >= 1) {
-            std::vector<int32_t> by_oct[8];
-            for (int32_t a = 0; a < lv.n_node(); ++a) by_oct[lv.code[a] & 7].push_back(a);
-            for (int o = 0; o < 8; ++o)
-                for (size_t b = 0; b < by_oct[o].size(); b += TB) {
-                    loct.push_back(o);
-                    for (int tt = 0; tt < TB; ++tt) {
-                        if (b + tt >= by_oct[o].size()) {
-                            lout.push_back(-1);
-                            lin.push_back(-1);
-                            continue;
-                        }
-                        const int32_t a = by_oct[o][b + tt];
-                        lout.push_back(static_cast<int32_t>(P->level_first[l] + a));
-                        lin.push_back(static_cast<int32_t>(P->level_first[l - 1] + lv.parent[a]));
-                    }
-                    lrow.push_back(static_cast<int64_t>(loct.size()));
-                }
+            for (int o = 0; o < 8; ++o) {
+                if (!present[o]) continue;
+                oct.push_back(o);
+                child.insert(child.end(), ch[o], ch[o] + TB);
+            }
+            row.push_back(static_cast<int64_t>(oct.size()));
         }
     }
     lv_m[t.depth + 1] = static_cast<int64_t>(row.size()) - 1;
-    lv_l[t.depth + 1] = static_cast<int64_t>(lrow.size()) - 1;
-    sfmm::TransArgs A, B;  // published together once every upload succeeded
+    for (int l = 1; l <= t.depth + 1; ++l) lv_l[l] = lv_m[l - 1];  // children at level l: parents at l-1
+    sfmm::TransArgs A;
     int rc;
-    if ((rc = P->upload(row, &A.blk_row)) || (rc = P->upload(out, &A.blk_out)) || (rc = P->upload(oct, &A.ent_oct)) ||
-        (rc = P->upload(in, &A.ent_in)) || (rc = P->upload(lrow, &B.blk_row)) || (rc = P->upload(lout, &B.blk_out)) ||
-        (rc = P->upload(loct, &B.ent_oct)) || (rc = P->upload(lin, &B.ent_in)))
+    if ((rc = P->upload(row, &A.blk_row)) || (rc = P->upload(parent_of_block, &A.blk_node)) ||
+        (rc = P->upload(oct, &A.ent_oct)) || (rc = P->upload(child, &A.ent_node)))
         return rc;
+    sfmm::TransArgs B = A;  // published together once every upload succeeded
     A.frag = P->m2m_frag;
     B.frag = P->l2l_frag;
     P->m2m[nc] = A;

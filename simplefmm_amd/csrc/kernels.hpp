@@ -123,10 +123,12 @@ struct LevelSpan {
 // (child octant, 16/nc input nodes); the operator of an octant is a dense
 // (p+1)^2 x (p+1)^2 matrix stored as MFMA A fragments.
 struct TransArgs {
-    const int64_t* blk_row = nullptr;  // [n_block+1] entry ranges
-    const int32_t* blk_out = nullptr;  // [n_block][16/nc] output gnodes, -1 = none
-    const int32_t* ent_oct = nullptr;  // child octant per entry
-    const int32_t* ent_in = nullptr;   // [entries][16/nc] input gnodes, -1 = none
+    // M2M and L2L share these lists: per block 16/nc parents, per entry the
+    // children of one octant of those parents
+    const int64_t* blk_row = nullptr;   // [n_block+1] entry ranges
+    const int32_t* blk_node = nullptr;  // [n_block][16/nc] parent gnodes, -1 = none
+    const int32_t* ent_oct = nullptr;   // child octant per entry
+    const int32_t* ent_node = nullptr;  // [entries][16/nc] child gnodes, -1 = none
     const double* frag = nullptr;      // [8][nfrag][64]
     float* out32 = nullptr;            // M2M (optional): FP32 copy of the parents written
 };
