@@ -58,10 +58,14 @@ __global__ void fill_kernel(double* x, int64_t count, double v)
 }
 
 // ------------------------------------------------------------------ P2M
-// One wavefront per leaf.  Lane (m, g) = (lane % (P+1), lane / (P+1)) sums
-// q * R(n,m) for n = m..P over the particles g, g+G, ... of the leaf, so no
-// cross-lane reduction is needed until the G partial sums are added.  The
-// multipole components are then formed from these plain moments S_c:
+// One wavefront per leaf.  Lane (u, g) = (lane % NU, lane / NU) carries the
+// azimuths ma = u and mb = P - u (one azimuth when they coincide) through the
+// degree recurrence and sums q * R(n,m) for n = m..P over the particles g,
+// g+G, ... of the leaf, so no cross-lane reduction is needed until the G
+// partial sums are added.  Pairing m with P - m gives every lane the same P+2
+// recurrence steps (one azimuth per lane would leave lane m idle for m of its
+// P+1 steps).  The multipole components are then formed from these plain
+// moments S_c:
 //   potential:        M_c = S_c
 //   potential+grad:   M_0 = S, M_d = -(1/w) D_d[S]
 //   curl:             M = (1/w) curl-like combination D x S (fmm.hpp:549-552)
@@ -76,7 +80,9 @@ __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int nc, co
 {
     constexpr int K = (P + 1) * (P + 1);
     constexpr int NT = (P + 1) * (P + 2) / 2;  // complex coefficients with m >= 0
-    constexpr int G = 64 / (P + 1);
+    constexpr int NU = P / 2 + 1;              // azimuth pairs
+    constexpr int G = 64 / NU;                 // particle groups
+    constexpr int NS = P + 2;                  // recurrence steps per lane
     constexpr int ncq = NCQ;
     __shared__ double part[G][NCQ][2 * NT];
     __shared__ double S[NCQ][2 * NT];
@@ -86,63 +92,85 @@ __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int nc, co
     const int32_t gnode = d.fl_gnode[f];
     const int64_t b = d.fl_begin[f], e = d.fl_end[f];
     const double invw = d.inv_width[d.fl_level[f]];
-    const int m = tid % (P + 1);
-    const int g = tid / (P + 1);
+    const int u = tid % NU, g = tid / NU;
+    const int ma = u, mb = P - u;
+    const int la = P - ma + 1;             // steps of azimuth ma; then those of mb
+    const int lb = mb > ma ? ma + 1 : 0;   // (none when the two coincide)
     const double* rec = d.rec;
-    // this lane's recurrence coefficients a(n, m), b(n, m), loaded once
-    double ra[P + 1], rb[P + 1];
+    // per step: degree n, azimuth m, whether it is one of this lane's terms,
+    // and the recurrence coefficients a(n, m), b(n, m) (0 at an azimuth's first step)
+    auto step_nm = [&](int t, int& n, int& m) {
+        if (t < la) {
+            n = ma + t;
+            m = ma;
+            return true;
+        }
+        n = mb + (t - la);
+        m = mb;
+        return t - la < lb;
+    };
+    double ra[NS], rb[NS];
 #pragma unroll
-    for (int n = 0; n <= P; ++n) {
-        ra[n] = n > m ? rec_a(rec, n, m) : 0.0;
-        rb[n] = n > m ? rec_b(rec, n, m) : 0.0;
+    for (int t = 0; t < NS; ++t) {
+        int n, m;
+        const bool use = step_nm(t, n, m) && t != 0 && t != la;
+        ra[t] = use ? rec_a(rec, n, m) : 0.0;
+        rb[t] = use ? rec_b(rec, n, m) : 0.0;
     }
 
     for (int cq = 0; cq < ncq; ++cq) {
-        double ar[P + 1], ai[P + 1];
+        double ar[NS], ai[NS];
 #pragma unroll
-        for (int n = 0; n <= P; ++n) ar[n] = ai[n] = 0.0;
+        for (int t = 0; t < NS; ++t) ar[t] = ai[t] = 0.0;
         if (g < G) {
             for (int64_t i = b + g; i < e; i += G) {
                 const double x = d.ux[i], y = d.uy[i], z = d.uz[i];
                 const double qi = q[cq * d.n + i];
                 const double r2 = x * x + y * y + z * z;
-                double dr = 1.0, di = 0.0;
+                // R(m,m) = d_m (-x + i y) R(m-1,m-1) up to mb, keeping R(ma,ma)
+                double dr = 1.0, di = 0.0, sar = 1.0, sai = 0.0;
 #pragma unroll
                 for (int k = 1; k <= P; ++k) {
-                    if (k <= m) {
+                    if (k <= mb) {
                         const double c = rec_d(rec, k);
                         const double tr = c * (-x * dr - y * di);
                         const double ti = c * (y * dr - x * di);
                         dr = tr;
                         di = ti;
                     }
-                }
-                double p1r = 0.0, p1i = 0.0, p2r = 0.0, p2i = 0.0;
-#pragma unroll
-                for (int n = 0; n <= P; ++n) {
-                    if (n == m) {
-                        p1r = dr;
-                        p1i = di;
-                        ar[n] += qi * dr;
-                        ai[n] += qi * di;
-                    } else if (n > m) {
-                        const double a = ra[n] * z, bb = rb[n] * r2;
-                        const double rr = a * p1r - bb * p2r;
-                        const double ri = a * p1i - bb * p2i;
-                        p2r = p1r;
-                        p2i = p1i;
-                        p1r = rr;
-                        p1i = ri;
-                        ar[n] += qi * rr;
-                        ai[n] += qi * ri;
+                    if (k == ma) {
+                        sar = dr;
+                        sai = di;
                     }
+                }
+                // R(n+1,m) = a z R(n,m) - b r2 R(n-1,m); b(m+1,m) = 0, so the
+                // value carried over from the previous azimuth drops out
+                double p1r = sar, p1i = sai, p2r = 0.0, p2i = 0.0;
+                ar[0] += qi * sar;
+                ai[0] += qi * sai;
+#pragma unroll
+                for (int t = 1; t < NS; ++t) {
+                    const double a = ra[t] * z, bb = rb[t] * r2;
+                    double rr = a * p1r - bb * p2r;
+                    double ri = a * p1i - bb * p2i;
+                    if (t == la) {  // first step of azimuth mb
+                        rr = dr;
+                        ri = di;
+                    }
+                    p2r = p1r;
+                    p2i = p1i;
+                    p1r = rr;
+                    p1i = ri;
+                    ar[t] += qi * rr;
+                    ai[t] += qi * ri;
                 }
             }
 #pragma unroll
-            for (int n = 0; n <= P; ++n) {
-                if (n >= m) {
-                    part[g][cq][2 * tri(n, m)] = ar[n];
-                    part[g][cq][2 * tri(n, m) + 1] = ai[n];
+            for (int t = 0; t < NS; ++t) {
+                int n, m;
+                if (step_nm(t, n, m)) {
+                    part[g][cq][2 * tri(n, m)] = ar[t];
+                    part[g][cq][2 * tri(n, m) + 1] = ai[t];
                 }
             }
         }
