@@ -42,13 +42,17 @@ __global__ void gather_charges_kernel(int64_t n, int qdim, const int32_t* __rest
     for (int c = 0; c < qdim; ++c) q[c * n + s] = Q[o * qdim + c];
 }
 
-__global__ void scatter_outputs_kernel(int64_t n, int nout, const int32_t* __restrict__ perm,
+// The sorted outputs are interleaved, out[slot * nout + c], so each particle's
+// values are one contiguous read; the caller's array is walked in its own
+// order (coalesced read-modify-write) and the sorted side gathered through the
+// inverse permutation.
+__global__ void scatter_outputs_kernel(int64_t n, int nout, const int32_t* __restrict__ iperm,
                                        const double* __restrict__ out, double* __restrict__ U)
 {
-    const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int64_t o = perm[s];
-    for (int c = 0; c < nout; ++c) U[o * nout + c] += out[c * n + s];
+    const int64_t o = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    const int64_t s = iperm[o];
+    for (int c = 0; c < nout; ++c) U[o * nout + c] += out[s * nout + c];
 }
 
 __global__ void fill_kernel(double* x, int64_t count, double v)
@@ -321,7 +325,7 @@ __global__ __launch_bounds__(256, P <= 12 ? 4 : 2) void l2p_kernel(DevPlan d, co
         }
     }
 #pragma unroll
-    for (int c = 0; c < NC; ++c) out[c * d.n + i] += invw * U[c];
+    for (int c = 0; c < NC; ++c) out[i * NC + c] += invw * U[c];
 }
 
 // ------------------------------------------------------------------ M2M / L2L
@@ -752,7 +756,7 @@ __global__ __launch_bounds__(64) void p2p_kernel(DevPlan d, P2PItems w, const do
                 for (int c = 0; c < NOUT; ++c) {
                     double v = 0.0;
                     for (int ss = 0; ss < S; ++ss) v += red[h][c][t + ss * U];
-                    if (pslot < 0) out[c * d.n + slot[h]] += v;
+                    if (pslot < 0) out[slot[h] * NOUT + c] += v;
                     else partial[(pslot + (slot[h] - tb)) * NOUT + c] = v;
                 }
             }
@@ -784,7 +788,7 @@ __global__ __launch_bounds__(64) void self_correction_kernel(DevPlan d, SelfCorr
         pair_kernel<MODE, NCQ>(xi - d.px[j], yi - d.py[j], zi - d.pz[j], qj, a);
     }
 #pragma unroll
-    for (int k = 0; k < NOUT; ++k) out[k * d.n + t] -= a[k];
+    for (int k = 0; k < NOUT; ++k) out[t * NOUT + k] -= a[k];
 }
 
 // Adds the partial sums of split leaves in item order.
@@ -798,7 +802,7 @@ __global__ void p2p_reduce_kernel(DevPlan d, P2PItems w, int nout, const double*
     for (int c = 0; c < nout; ++c) {
         double v = 0.0;
         for (int64_t i = 0; i < nitem; ++i) v += partial[(first + i * stride) * nout + c];
-        out[c * d.n + slot] += v;
+        out[slot * nout + c] += v;
     }
 }
 
@@ -942,7 +946,7 @@ void launch_gather_charges(const DevPlan& d, int qdim, const double* Q, double* 
 
 void launch_scatter_outputs(const DevPlan& d, int nout, const double* out, double* U, hipStream_t s)
 {
-    hipLaunchKernelGGL(scatter_outputs_kernel, dim3(grid_for(d.n, 256)), dim3(256), 0, s, d.n, nout, d.perm, out, U);
+    hipLaunchKernelGGL(scatter_outputs_kernel, dim3(grid_for(d.n, 256)), dim3(256), 0, s, d.n, nout, d.iperm, out, U);
 }
 
 void launch_fill(double* x, int64_t count, double v, hipStream_t s)
