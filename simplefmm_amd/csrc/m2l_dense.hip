@@ -15,19 +15,23 @@ void launch_m2l_dense(const DenseArgs& a, int p, int nc, const float* M32, doubl
 
 namespace {
 
-// One wavefront per node: lanes stride over its nc x kd floats, keep the
-// largest |value| per component, and reduce across the wavefront.
+// One wavefront per node: lanes stride over its nc x kd floats four at a time
+// (kd is a multiple of 64, so every row is 16-byte aligned), keep the largest
+// |value| per component, and reduce across the wavefront.
 __global__ __launch_bounds__(256) void nodemax_kernel(const float* __restrict__ M32, int64_t count, int nc, int kd,
                                                       float* __restrict__ nodemax)
 {
     const int lane = threadIdx.x & 63;
+    const int kd4 = kd / 4;
     for (int64_t node = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / 64; node < count;
          node += static_cast<int64_t>(gridDim.x) * blockDim.x / 64) {
-        const float* x = M32 + node * nc * kd;
+        const float4* x = reinterpret_cast<const float4*>(M32 + node * nc * kd);
         float mx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int i = lane; i < nc * kd; i += 64) {
-            const int c = i / kd;
-            mx[c] = fmaxf(mx[c], fabsf(x[i]));
+        for (int i = lane; i < nc * kd4; i += 64) {
+            const int c = i / kd4;
+            const float4 v = x[i];
+            const float m = fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w)));
+            mx[c] = fmaxf(mx[c], m);
         }
         for (int c = 0; c < nc; ++c)
             for (int o = 32; o > 0; o >>= 1) mx[c] = fmaxf(mx[c], __shfl_xor(mx[c], o));
