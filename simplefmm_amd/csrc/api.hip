@@ -340,8 +340,8 @@ int ensure_buffers(sfmm_plan_s* P, int nc, int qdim)
         if (dev_malloc(&P->M, bytes) != hipSuccess || dev_malloc(&P->L, bytes) != hipSuccess)
             return fail(SFMM_ERR_ALLOC, "hipMalloc failed (expansions)");
         if (P->precision == SFMM_MIXED && P->m2l_kernel == 0) {
-            // the dense layout pads each (node, component) to dense_kd floats; the
-            // padding stays zero (P2M and M2M write only the coefficients)
+            // the dense layout pads each (node, component) to dense_kd floats
+            // (the FP32 copy kernel writes the padding as zeros)
             const int row = P->dense ? P->dense_kd : P->K;
             const size_t b32 = sizeof(float) * static_cast<size_t>(nnode) * nc * row;
             if (dev_malloc(&P->M32, b32) != hipSuccess) return fail(SFMM_ERR_ALLOC, "hipMalloc failed (FP32 multipoles)");
@@ -666,12 +666,12 @@ int run_on(sfmm_plan_s* P, sfmm::Mode mode, int nc, int qdim, const double* dQ, 
     sfmm::launch_fill(P->obuf, static_cast<int64_t>(nc) * d.n, 0.0, s);
     REC(1);
     const bool far = (which & SFMM_FAR) && depth >= 2;  // fmm.hpp:816-820
-    // the mixed M2L reads an FP32 copy of the multipoles, written by P2M and M2M
+    // the mixed M2L reads an FP32 copy of the multipoles (levels >= 2), made in
+    // one pass before it (P2M and M2M write only the FP64 multipoles)
     float* M32 = P->m2l_kernel == 0 && (P->dense || P->m2l[nc].fp32) ? P->M32 : nullptr;
-    if (far) sfmm::launch_p2m(d, mode, nc, P->qbuf, P->M, M32, s);
+    if (far) sfmm::launch_p2m(d, mode, nc, P->qbuf, P->M, s);
     REC(2);
-    sfmm::TransArgs m2m = P->m2m[nc];
-    m2m.out32 = M32;
+    const sfmm::TransArgs& m2m = P->m2m[nc];
     if (far)
         for (int l = depth - 1; l >= 2; --l) {
             if (P->m2l_kernel == 0)
@@ -681,11 +681,14 @@ int run_on(sfmm_plan_s* P, sfmm::Mode mode, int nc, int qdim, const double* dQ, 
         }
     REC(3);
     if (far) {
-        if (P->dense) {
-            // range of every FP32 multipole (levels >= 2), which scales the columns, then M2L
+        if (M32) {
+            // and, for the dense M2L, the range of every FP32 multipole, which scales the columns
             const int64_t f2 = P->level_first[2];
-            sfmm::launch_dense_nodemax(M32 + f2 * nc * P->dense_kd, P->level_first.back() - f2, nc, P->dense_kd,
-                                       P->dense_nodemax + f2 * nc, s);
+            const int row = P->dense ? P->dense_kd : P->K;
+            sfmm::launch_fp32_copy(P->M + f2 * nc * P->K, P->level_first.back() - f2, nc, P->p, M32 + f2 * nc * row,
+                                   P->dense ? P->dense_nodemax + f2 * nc : nullptr, s);
+        }
+        if (P->dense) {
             sfmm::launch_m2l_dense(P->dns[nc], P->p, nc, M32, P->L, s);
         } else if (P->m2l_kernel == 0) {
             sfmm::M2LArgs& A = P->m2l[nc];

@@ -80,7 +80,7 @@ __global__ void fill_kernel(double* x, int64_t count, double v)
 //    dz         R(n,m) =  sqrt((n-m)(n+m))   R(n-1,m)
 template <int P, int NCQ>
 __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int nc, const double* __restrict__ q,
-                                                 double* __restrict__ M, float* __restrict__ M32)
+                                                 double* __restrict__ M)
 {
     constexpr int K = (P + 1) * (P + 1);
     constexpr int NT = (P + 1) * (P + 2) / 2;  // complex coefficients with m >= 0
@@ -189,12 +189,7 @@ __global__ __launch_bounds__(64) void p2m_kernel(DevPlan d, int mode, int nc, co
     __syncthreads();
 
     double* Mg = M + static_cast<int64_t>(gnode) * nc * K;
-    using FL = F32LayoutOf<P>;  // FP32 copy for the mixed M2L
-    float* Mg32 = M32 ? M32 + static_cast<int64_t>(gnode) * nc * FL::L.row : nullptr;
-    auto put = [&](int c, int r, double v) {
-        Mg[c * K + r] = v;
-        if (Mg32) Mg32[c * FL::L.row + FL::L.pos[r]] = static_cast<float>(v);
-    };
+    auto put = [&](int c, int r, double v) { Mg[c * K + r] = v; };
     for (int t = tid; t < NT; t += 64) {
         // decode (n, mm) of triangle index t
         int n = 0;
@@ -904,15 +899,15 @@ void with_p(int p, A&&... args)
 
 template <int P>
 struct P2MRun {
-    static void run(const DevPlan& d, int mode, int ncq, int nc, const double* q, double* M, float* M32,
+    static void run(const DevPlan& d, int mode, int ncq, int nc, const double* q, double* M,
                     hipStream_t s)
     {
         const dim3 grid(static_cast<unsigned>(d.n_far_leaf)), blk(64);
         switch (ncq) {
-        case 1: hipLaunchKernelGGL((p2m_kernel<P, 1>), grid, blk, 0, s, d, mode, nc, q, M, M32); break;
-        case 2: hipLaunchKernelGGL((p2m_kernel<P, 2>), grid, blk, 0, s, d, mode, nc, q, M, M32); break;
-        case 3: hipLaunchKernelGGL((p2m_kernel<P, 3>), grid, blk, 0, s, d, mode, nc, q, M, M32); break;
-        default: hipLaunchKernelGGL((p2m_kernel<P, 4>), grid, blk, 0, s, d, mode, nc, q, M, M32); break;
+        case 1: hipLaunchKernelGGL((p2m_kernel<P, 1>), grid, blk, 0, s, d, mode, nc, q, M); break;
+        case 2: hipLaunchKernelGGL((p2m_kernel<P, 2>), grid, blk, 0, s, d, mode, nc, q, M); break;
+        case 3: hipLaunchKernelGGL((p2m_kernel<P, 3>), grid, blk, 0, s, d, mode, nc, q, M); break;
+        default: hipLaunchKernelGGL((p2m_kernel<P, 4>), grid, blk, 0, s, d, mode, nc, q, M); break;
         }
     }
 };
@@ -954,11 +949,11 @@ void launch_fill(double* x, int64_t count, double v, hipStream_t s)
     if (count > 0) hipLaunchKernelGGL(fill_kernel, dim3(grid_for(count, 256)), dim3(256), 0, s, x, count, v);
 }
 
-void launch_p2m(const DevPlan& d, Mode mode, int nc, const double* q, double* M, float* M32, hipStream_t s)
+void launch_p2m(const DevPlan& d, Mode mode, int nc, const double* q, double* M, hipStream_t s)
 {
     if (d.n_far_leaf == 0) return;
     const int ncq = mode == MODE_POT ? nc : (mode == MODE_CURL ? 3 : 1);
-    with_p<P2MRun>(d.p, d, static_cast<int>(mode), ncq, nc, q, M, M32, s);
+    with_p<P2MRun>(d.p, d, static_cast<int>(mode), ncq, nc, q, M, s);
 }
 
 void launch_l2p(const DevPlan& d, int nc, const double* L, double* out, hipStream_t s)

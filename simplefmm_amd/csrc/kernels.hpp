@@ -130,14 +130,12 @@ struct TransArgs {
     const int32_t* ent_oct = nullptr;   // child octant per entry
     const int32_t* ent_node = nullptr;  // [entries][16/nc] child gnodes, -1 = none
     const double* frag = nullptr;      // [8][nfrag][64]
-    float* out32 = nullptr;            // M2M (optional): FP32 copy of the parents written
 };
 
 // All launches are asynchronous on `stream`.
 void launch_gather_charges(const DevPlan& d, int qdim, const double* Q, double* q, hipStream_t s);
 void launch_scatter_outputs(const DevPlan& d, int nout, const double* out, double* U, hipStream_t s);
-// M32 (optional): an FP32 copy of the leaf multipoles, written alongside M
-void launch_p2m(const DevPlan& d, Mode mode, int nc, const double* q, double* M, float* M32, hipStream_t s);
+void launch_p2m(const DevPlan& d, Mode mode, int nc, const double* q, double* M, hipStream_t s);
 void launch_m2m(const DevPlan& d, LevelSpan parents, int nc, double* M, hipStream_t s);
 void launch_m2l(const DevPlan& d, int nc, const double* M, double* L, hipStream_t s);
 void launch_l2l(const DevPlan& d, LevelSpan children, int nc, double* L, hipStream_t s);
@@ -145,8 +143,10 @@ void launch_l2p(const DevPlan& d, int nc, const double* L, double* out, hipStrea
 void launch_m2l_mfma(const M2LArgs& a, int p, int nc, const double* M, double* L, hipStream_t s);
 // p <= 12; M32: the multipoles in FP32, [gnode][nc][dense_order(p).kd] (m2l_dense.hpp)
 void launch_m2l_dense(const DenseArgs& a, int p, int nc, const float* M32, double* L, hipStream_t s);
-// nodemax[node * nc + c] = largest |M32| of component c of each of `count` nodes of nc x kd floats
-void launch_dense_nodemax(const float* M32, int64_t count, int nc, int kd, float* nodemax, hipStream_t s);
+// The FP32 copy (F32LayoutOf<p>: the dense M2L's layout for p <= 12, else the
+// real-form order) of `count` nodes of nc x (p+1)^2 FP64 coefficients, with
+// their per-component largest |value| into nodemax when it is not null.
+void launch_fp32_copy(const double* M, int64_t count, int nc, int p, float* M32, float* nodemax, hipStream_t s);
 // upper = false: M2M (parents written); true: L2L (children added to).
 void launch_translate(const TransArgs& a, LevelSpan blocks, int p, int nc, bool upper, double* io, hipStream_t s);
 int translate_nfrag(int p, bool upper);

@@ -239,14 +239,7 @@ __device__ __forceinline__ void translate_wave(const TransArgs& a, int64_t blk, 
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * mt + 4 * r + (lane >> 4);  // f64 16x16x4 D layout
-                if (row < K) {
-                    out[row] = acc[q][r];
-                    if (a.out32) {
-                        using FL = F32LayoutOf<P>;
-                        a.out32[(static_cast<int64_t>(dst) * NC + c) * FL::L.row + FL::L.pos[row]] =
-                            static_cast<float>(acc[q][r]);
-                    }
-                }
+                if (row < K) out[row] = acc[q][r];
             }
         });
     }
